@@ -1,0 +1,1094 @@
+// GpuTlmHip.hip - C ABI of the gfx950 TLM step (include/GpuTlmHip.h).
+//
+// Replaces the OpenGL set-up and per-step dispatch of GpuTlm::run
+// (GpuTlm.cpp:532-884 in the reference) with HIP: one stream per context,
+// device-resident per-step tables, three launches per step (inject,
+// monitors, fused scatter/connect) and no host synchronisation inside a run.
+#include "../../include/GpuTlmHip.h"
+#include "tlm_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_create_error = "no error";
+
+struct DevBuf
+{
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+} // namespace
+
+struct tlm_ctx
+{
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evk0 = nullptr, evk1 = nullptr;
+    int nx = 0, ny = 0, nz = 0, px = 0;
+    long long P = 0;
+    float dl = 0.f, rho = 0.f;
+    // launch shape of the fused kernel
+    int Q = 0, R = 0, zc = 0, gx = 0, gy = 0, threads = 0;
+    // state
+    float *A = nullptr, *B = nullptr, *Vs = nullptr;
+    uint8_t *mat = nullptr;
+    float4 *props = nullptr;
+    int nmat = 0;
+    bool stubs = false, uploaded = false;
+    double *epart = nullptr;
+    // sources
+    long long *ucell = nullptr;
+    int *ubeg = nullptr, *spol = nullptr;
+    float *samp = nullptr;
+    int nuniq = 0;
+    // schedule
+    float *sv = nullptr, *cph = nullptr, *sph = nullptr;
+    int nsteps = 0, settle = 0;
+    // monitors
+    long long *fcell = nullptr;
+    int *faxis = nullptr;
+    float *js_inst = nullptr, *js_dft4 = nullptr;
+    int nf = 0;
+    long long *hcell = nullptr;
+    int *haxis = nullptr;
+    float *huy8 = nullptr;
+    int nh = 0;
+    long long *pcells = nullptr, *ploop = nullptr;
+    int *poff = nullptr, *ppol = nullptr;
+    float *prec = nullptr;
+    int np = 0;
+    long long *prcell = nullptr;
+    float inv_dl = 0.f;
+    float *prrec = nullptr;
+    int npr = 0;
+    double *ctab = nullptr, *stab = nullptr, *dft_re = nullptr, *dft_im = nullptr;
+    int nfreq = 0;
+    int dft_done = 0; // probe DFT accumulated over steps [0, dft_done)
+    // energy
+    int every = 16;
+    double *energy = nullptr;
+    std::vector<int> energy_steps;
+    int steps_run_hi = 0; // highest step index + 1 queued so far
+    std::string err = "no error";
+    std::vector<void *> owned;
+};
+
+namespace {
+
+int fail(tlm_ctx *c, int code, const std::string &msg)
+{
+    if (c)
+        c->err = msg;
+    else
+        g_create_error = msg;
+    return code;
+}
+
+int hip_fail(tlm_ctx *c, hipError_t e, const char *what)
+{
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    return fail(c, e == hipErrorOutOfMemory ? TLM_ERR_NOMEM : TLM_ERR_HIP, m);
+}
+
+#define HIPCHK(ctx, call)                                                                          \
+    do                                                                                             \
+    {                                                                                              \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return hip_fail((ctx), e_, #call);                                                     \
+    } while (0)
+
+template <class T> int dalloc(tlm_ctx *c, T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0)
+        count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess)
+    {
+        *p = nullptr;
+        return hip_fail(c, e, "hipMalloc");
+    }
+    c->owned.push_back(*p);
+    return TLM_OK;
+}
+
+template <class T> void dfree(tlm_ctx *c, T *&p)
+{
+    if (!p)
+        return;
+    auto it = std::find(c->owned.begin(), c->owned.end(), (void *)p);
+    if (it != c->owned.end())
+        c->owned.erase(it);
+    (void)hipFree(p);
+    p = nullptr;
+}
+
+template <class T> int upload(tlm_ctx *c, T **dst, const T *src, size_t count)
+{
+    dfree(c, *dst);
+    int rc = dalloc(c, dst, count);
+    if (rc)
+        return rc;
+    if (count && src)
+        HIPCHK(c, hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice));
+    return TLM_OK;
+}
+
+long long padded_cell(const tlm_ctx *c, long long cell)
+{
+    long long i = cell % c->nx, jk = cell / c->nx;
+    long long j = jk % c->ny, k = jk / c->ny;
+    return (k * c->ny + j) * c->px + i;
+}
+
+bool cell_ok(const tlm_ctx *c, long long cell)
+{
+    return cell >= 0 && cell < (long long)c->nx * c->ny * c->nz;
+}
+
+// Launch shape: each workgroup holds R whole rows (Q quads each) and
+// marches over zc planes. zc is chosen so the grid has enough workgroups to
+// fill 256 CUs several times over.
+void choose_shape(tlm_ctx *c)
+{
+    c->Q = c->px / 4;
+    c->R = std::max(1, tlm::kMaxThreads / c->Q);
+    c->R = std::min(c->R, c->ny);
+    c->threads = ((c->Q * c->R + 63) / 64) * 64;
+    c->gx = (c->ny + c->R - 1) / c->R;
+    const int target = 2048;
+    int nchunks = std::max(1, std::min(c->nz, (target + c->gx - 1) / c->gx));
+    c->zc = (c->nz + nchunks - 1) / nchunks;
+    c->gy = (c->nz + c->zc - 1) / c->zc;
+}
+
+int launch_step(tlm_ctx *c, int n)
+{
+    tlm::StepArgs a;
+    a.A = c->A;
+    a.B = c->B;
+    a.Vs = c->Vs;
+    a.mat = c->mat;
+    a.props = c->props;
+    a.epart = c->epart;
+    a.nx = c->nx;
+    a.ny = c->ny;
+    a.nz = c->nz;
+    a.px = c->px;
+    a.Q = c->Q;
+    a.R = c->R;
+    a.zc = c->zc;
+    a.P = c->P;
+    a.rho = c->rho;
+    const bool energy = c->every > 0 && (n % c->every) == 0;
+    dim3 grid(c->gx, c->gy), block(c->threads);
+    if (c->stubs)
+    {
+        if (energy)
+            hipLaunchKernelGGL((tlm::k_step<true, true>), grid, block, 0, c->stream, a);
+        else
+            hipLaunchKernelGGL((tlm::k_step<true, false>), grid, block, 0, c->stream, a);
+    }
+    else
+    {
+        if (energy)
+            hipLaunchKernelGGL((tlm::k_step<false, true>), grid, block, 0, c->stream, a);
+        else
+            hipLaunchKernelGGL((tlm::k_step<false, false>), grid, block, 0, c->stream, a);
+    }
+    return TLM_OK;
+}
+
+int queue_steps(tlm_ctx *c, int step0, int nsteps, bool time_kernel, float *ms_kernel)
+{
+    if (!c->uploaded)
+        return fail(c, TLM_ERR_STATE, "tlm_run before tlm_upload");
+    if (nsteps < 0 || step0 < 0)
+        return fail(c, TLM_ERR_ARG, "negative step range");
+    if (nsteps == 0)
+        return TLM_OK;
+    if (!c->sv || step0 + nsteps > c->nsteps)
+        return fail(c, TLM_ERR_STATE, "step range exceeds the schedule set by tlm_set_schedule");
+    HIPCHK(c, hipSetDevice(c->dev));
+    const int nmon = c->nf + c->nh + c->np + c->npr;
+    float kern_ms = 0.f;
+    for (int n = step0; n < step0 + nsteps; ++n)
+    {
+        if (c->nuniq > 0)
+        {
+            int b = 128, g = (c->nuniq + b - 1) / b;
+            hipLaunchKernelGGL(tlm::k_inject, dim3(g), dim3(b), 0, c->stream, c->A, c->P, c->ucell,
+                               c->ubeg, c->spol, c->samp, c->nuniq, c->sv, n);
+        }
+        if (nmon > 0)
+        {
+            tlm::MonitorArgs m;
+            m.A = c->A;
+            m.P = c->P;
+            m.n = n;
+            m.settle = c->settle;
+            m.cph = c->cph;
+            m.sph = c->sph;
+            m.nf = c->nf;
+            m.fcell = c->fcell;
+            m.faxis = c->faxis;
+            m.js_inst = c->js_inst;
+            m.js_dft4 = c->js_dft4;
+            m.nh = c->nh;
+            m.hcell = c->hcell;
+            m.haxis = c->haxis;
+            m.huy8 = c->huy8;
+            m.np = c->np;
+            m.pcells = c->pcells;
+            m.poff = c->poff;
+            m.ppol = c->ppol;
+            m.ploop = c->ploop;
+            m.prec = c->prec;
+            m.npr = c->npr;
+            m.prcell = c->prcell;
+            m.inv_dl = c->inv_dl;
+            m.prrec = c->prrec;
+            int b = 256, g = (nmon + b - 1) / b;
+            hipLaunchKernelGGL(tlm::k_monitor, dim3(g), dim3(b), 0, c->stream, m);
+        }
+        if (time_kernel)
+            HIPCHK(c, hipEventRecord(c->evk0, c->stream));
+        launch_step(c, n);
+        if (time_kernel)
+        {
+            HIPCHK(c, hipEventRecord(c->evk1, c->stream));
+            HIPCHK(c, hipEventSynchronize(c->evk1));
+            float ms = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&ms, c->evk0, c->evk1));
+            kern_ms += ms;
+        }
+        if (c->every > 0 && (n % c->every) == 0)
+        {
+            const int idx = n / c->every;
+            hipLaunchKernelGGL(tlm::k_energy_reduce, dim3(1), dim3(256), 0, c->stream, c->epart,
+                               c->gx * c->gy, c->energy + idx);
+            if ((int)c->energy_steps.size() <= idx)
+                c->energy_steps.resize(idx + 1, -1);
+            c->energy_steps[idx] = n;
+        }
+        std::swap(c->A, c->B);
+    }
+    HIPCHK(c, hipGetLastError());
+    // running probe DFT over the newly recorded steps, in step order
+    if (c->npr > 0 && c->nfreq > 0)
+    {
+        if (step0 != c->dft_done)
+            return fail(c, TLM_ERR_STATE, "probe DFT needs contiguous step blocks from step 0");
+        int tot = c->npr * 3 * c->nfreq, b = 128;
+        hipLaunchKernelGGL(tlm::k_probe_dft, dim3((tot + b - 1) / b), dim3(b), 0, c->stream,
+                           c->prrec, c->npr, c->ctab, c->stab, c->nfreq, step0, step0 + nsteps,
+                           c->dft_re, c->dft_im);
+        c->dft_done = step0 + nsteps;
+    }
+    c->steps_run_hi = std::max(c->steps_run_hi, step0 + nsteps);
+    if (ms_kernel)
+        *ms_kernel = kern_ms;
+    HIPCHK(c, hipGetLastError());
+    return TLM_OK;
+}
+
+// Chunked AoS <-> SoA conversion through a bounded staging buffer.
+int convert(tlm_ctx *c, const float *host_in, float *host_out, float *soa, int ncomp)
+{
+    const long long ncell = (long long)c->nx * c->ny * c->nz;
+    const long long chunk = std::min<long long>(ncell, (long long)8 << 20); // 8 M cells
+    float *stage = nullptr;
+    int rc = dalloc(c, &stage, (size_t)chunk * ncomp);
+    if (rc)
+        return rc;
+    for (long long c0 = 0; c0 < ncell; c0 += chunk)
+    {
+        long long cnt = std::min(chunk, ncell - c0);
+        int b = 256;
+        long long g = (cnt + b - 1) / b;
+        if (host_in)
+        {
+            hipError_t e = hipMemcpyAsync(stage, host_in + c0 * ncomp, cnt * ncomp * sizeof(float),
+                                          hipMemcpyHostToDevice, c->stream);
+            if (e != hipSuccess)
+            {
+                dfree(c, stage);
+                return hip_fail(c, e, "hipMemcpyAsync");
+            }
+            hipLaunchKernelGGL(tlm::k_aos_to_soa, dim3((unsigned)g), dim3(b), 0, c->stream, stage,
+                               soa, c->P, ncomp, c->nx, c->ny, c->px, c0, cnt);
+        }
+        else
+        {
+            hipLaunchKernelGGL(tlm::k_soa_to_aos, dim3((unsigned)g), dim3(b), 0, c->stream, soa,
+                               stage, c->P, ncomp, c->nx, c->ny, c->px, c0, cnt);
+            hipError_t e = hipMemcpyAsync(host_out + c0 * ncomp, stage, cnt * ncomp * sizeof(float),
+                                          hipMemcpyDeviceToHost, c->stream);
+            if (e != hipSuccess)
+            {
+                dfree(c, stage);
+                return hip_fail(c, e, "hipMemcpyAsync");
+            }
+        }
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess)
+        {
+            dfree(c, stage);
+            return hip_fail(c, e, "hipStreamSynchronize");
+        }
+    }
+    dfree(c, stage);
+    return TLM_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int tlm_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess)
+        return 0;
+    return n;
+}
+
+const char *tlm_device_name(int device)
+{
+    static thread_local char name[256];
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, device) != hipSuccess)
+        return "unknown adapter";
+    std::snprintf(name, sizeof(name), "%s (%s)", p.name, p.gcnArchName);
+    return name;
+}
+
+int tlm_create(const tlm_grid_desc *d, tlm_ctx **out)
+{
+    if (!out)
+        return fail(nullptr, TLM_ERR_ARG, "tlm_create: out is NULL");
+    *out = nullptr;
+    if (!d)
+        return fail(nullptr, TLM_ERR_ARG, "tlm_create: desc is NULL");
+    if (d->nx <= 0 || d->ny <= 0 || d->nz <= 0)
+        return fail(nullptr, TLM_ERR_ARG, "empty grid");
+    if (d->pulse_type != TLM_PULSE_F32)
+        return fail(nullptr, TLM_ERR_UNSUPPORTED, "only FP32 pulse storage in this entry point");
+    if (((d->nx + 3) / 4) > tlm::kMaxThreads)
+        return fail(nullptr, TLM_ERR_UNSUPPORTED, "nx above 1024 cells is not supported");
+    if (!(d->dl > 0.0f))
+        return fail(nullptr, TLM_ERR_ARG, "cell size must be positive");
+    int ndev = tlm_device_count();
+    if (ndev <= 0)
+        return fail(nullptr, TLM_ERR_NO_DEVICE, "no HIP device available");
+    if (d->device < 0 || d->device >= ndev)
+        return fail(nullptr, TLM_ERR_ARG, "device ordinal out of range");
+
+    tlm_ctx *c = new tlm_ctx;
+    c->dev = d->device;
+    c->nx = d->nx;
+    c->ny = d->ny;
+    c->nz = d->nz;
+    c->px = (d->nx + 3) / 4 * 4;
+    c->P = (long long)c->px * c->ny * c->nz;
+    c->dl = d->dl;
+    c->rho = d->rho;
+    c->inv_dl = -1.0f / d->dl;
+    choose_shape(c);
+    auto bail = [&](int rc) {
+        g_create_error = c->err;
+        tlm_destroy(c);
+        return rc;
+    };
+    hipError_t e = hipSetDevice(c->dev);
+    if (e != hipSuccess)
+        return bail(hip_fail(c, e, "hipSetDevice"));
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
+        return bail(hip_fail(c, e, "hipStreamCreate"));
+    for (hipEvent_t *ev : {&c->ev0, &c->ev1, &c->evk0, &c->evk1})
+        if ((e = hipEventCreate(ev)) != hipSuccess)
+            return bail(hip_fail(c, e, "hipEventCreate"));
+    int rc;
+    if ((rc = dalloc(c, &c->A, (size_t)c->P * 12)) || (rc = dalloc(c, &c->B, (size_t)c->P * 12)) ||
+        (rc = dalloc(c, &c->mat, (size_t)c->P)) ||
+        (rc = dalloc(c, &c->epart, (size_t)c->gx * c->gy)))
+        return bail(rc);
+    *out = c;
+    return TLM_OK;
+}
+
+void tlm_destroy(tlm_ctx *c)
+{
+    if (!c)
+        return;
+    (void)hipSetDevice(c->dev);
+    if (c->stream)
+        (void)hipStreamSynchronize(c->stream);
+    for (void *p : c->owned)
+        (void)hipFree(p);
+    c->owned.clear();
+    for (hipEvent_t ev : {c->ev0, c->ev1, c->evk0, c->evk1})
+        if (ev)
+            (void)hipEventDestroy(ev);
+    if (c->stream)
+        (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *tlm_last_error(const tlm_ctx *c)
+{
+    return c ? c->err.c_str() : g_create_error.c_str();
+}
+
+int tlm_upload(tlm_ctx *c, const uint8_t *mat, const float *yhat, const float *ghat, int nmat,
+               const float *V, const float *Vs)
+{
+    if (!c || !mat || nmat < 0 || (nmat > 0 && (!yhat || !ghat)))
+        return fail(c, TLM_ERR_ARG, "tlm_upload: bad arguments");
+    HIPCHK(c, hipSetDevice(c->dev));
+    const long long ncell = (long long)c->nx * c->ny * c->nz;
+    bool any_diel = false;
+    int max_id = -1;
+    for (long long i = 0; i < ncell; ++i)
+        if (mat[i] >= 2)
+        {
+            any_diel = true;
+            max_id = std::max(max_id, (int)mat[i] - 2);
+        }
+    if (max_id >= nmat)
+        return fail(c, TLM_ERR_ARG, "material id without a yhat/ghat entry");
+    // padded material map: padding cells are PEC so they never link
+    std::vector<uint8_t> pm((size_t)c->P, 1);
+    for (long long k = 0; k < c->nz; ++k)
+        for (long long j = 0; j < c->ny; ++j)
+            std::memcpy(&pm[(size_t)((k * c->ny + j) * c->px)], mat + (k * c->ny + j) * c->nx,
+                        (size_t)c->nx);
+    HIPCHK(c, hipMemcpy(c->mat, pm.data(), pm.size(), hipMemcpyHostToDevice));
+    // per-material (yh, gh, den), den = 4 + yh + gh as in TlmSolver.cpp:90
+    std::vector<float4> props(std::max(1, nmat));
+    for (int i = 0; i < nmat; ++i)
+    {
+        float den = 4.0f + yhat[i] + ghat[i];
+        props[i] = make_float4(yhat[i], ghat[i], den, 0.f);
+    }
+    int rc = upload(c, &c->props, props.data(), props.size());
+    if (rc)
+        return rc;
+    c->nmat = nmat;
+    c->stubs = any_diel;
+    HIPCHK(c, hipMemsetAsync(c->A, 0, (size_t)c->P * 12 * sizeof(float), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->B, 0, (size_t)c->P * 12 * sizeof(float), c->stream));
+    dfree(c, c->Vs);
+    if (c->stubs)
+    {
+        if ((rc = dalloc(c, &c->Vs, (size_t)c->P * 3)))
+            return rc;
+        HIPCHK(c, hipMemsetAsync(c->Vs, 0, (size_t)c->P * 3 * sizeof(float), c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (V && (rc = convert(c, V, nullptr, c->A, 12)))
+        return rc;
+    if (Vs && c->stubs && (rc = convert(c, Vs, nullptr, c->Vs, 3)))
+        return rc;
+    c->uploaded = true;
+    c->energy_steps.clear();
+    c->steps_run_hi = 0;
+    c->dft_done = 0;
+    return TLM_OK;
+}
+
+int tlm_set_sources(tlm_ctx *c, const int64_t *cells, const int32_t *pol, const float *amp, int n)
+{
+    if (!c || n < 0 || (n > 0 && (!cells || !pol || !amp)))
+        return fail(c, TLM_ERR_ARG, "tlm_set_sources: bad arguments");
+    // group by cell, keeping list order within a cell
+    std::map<long long, std::vector<int>> by;
+    for (int s = 0; s < n; ++s)
+    {
+        if (!cell_ok(c, cells[s]) || pol[s] < 0 || pol[s] > 2)
+            return fail(c, TLM_ERR_ARG, "source cell or polarisation out of range");
+        by[cells[s]].push_back(s);
+    }
+    std::vector<long long> uc;
+    std::vector<int> ub, sp;
+    std::vector<float> sa;
+    for (auto &kv : by)
+    {
+        uc.push_back(padded_cell(c, kv.first));
+        ub.push_back((int)sp.size());
+        for (int s : kv.second)
+        {
+            sp.push_back(pol[s]);
+            sa.push_back(amp[s]);
+        }
+    }
+    ub.push_back((int)sp.size());
+    int rc;
+    if ((rc = upload(c, &c->ucell, uc.data(), uc.size())) ||
+        (rc = upload(c, &c->ubeg, ub.data(), ub.size())) ||
+        (rc = upload(c, &c->spol, sp.data(), sp.size())) ||
+        (rc = upload(c, &c->samp, sa.data(), sa.size())))
+        return rc;
+    c->nuniq = (int)uc.size();
+    return TLM_OK;
+}
+
+int tlm_set_schedule(tlm_ctx *c, const float *sv, const float *cph, const float *sph, int nsteps,
+                     int settle)
+{
+    if (!c || nsteps <= 0 || !sv || !cph || !sph)
+        return fail(c, TLM_ERR_ARG, "tlm_set_schedule: bad arguments");
+    int rc;
+    if ((rc = upload(c, &c->sv, sv, nsteps)) || (rc = upload(c, &c->cph, cph, nsteps)) ||
+        (rc = upload(c, &c->sph, sph, nsteps)))
+        return rc;
+    c->nsteps = nsteps;
+    c->settle = settle;
+    c->nfreq = 0; // DFT tables are sized by the schedule: set them again
+    c->dft_done = 0;
+    const int ne = c->every > 0 ? nsteps / c->every + 1 : 1;
+    dfree(c, c->energy);
+    if ((rc = dalloc(c, &c->energy, ne)))
+        return rc;
+    c->energy_steps.clear();
+    // record buffers that scale with the schedule length
+    if (c->np > 0)
+    {
+        dfree(c, c->prec);
+        if ((rc = dalloc(c, &c->prec, (size_t)nsteps * c->np * 2)))
+            return rc;
+        HIPCHK(c, hipMemset(c->prec, 0, (size_t)nsteps * c->np * 2 * sizeof(float)));
+    }
+    if (c->npr > 0)
+    {
+        dfree(c, c->prrec);
+        if ((rc = dalloc(c, &c->prrec, (size_t)nsteps * c->npr * 3)))
+            return rc;
+        HIPCHK(c, hipMemset(c->prrec, 0, (size_t)nsteps * c->npr * 3 * sizeof(float)));
+    }
+    return TLM_OK;
+}
+
+int tlm_set_energy_every(tlm_ctx *c, int every)
+{
+    if (!c || every < 0)
+        return fail(c, TLM_ERR_ARG, "tlm_set_energy_every: bad arguments");
+    c->every = every;
+    if (c->nsteps > 0)
+    {
+        dfree(c, c->energy);
+        int rc = dalloc(c, &c->energy, every > 0 ? c->nsteps / every + 1 : 1);
+        if (rc)
+            return rc;
+    }
+    c->energy_steps.clear();
+    return TLM_OK;
+}
+
+int tlm_set_probes(tlm_ctx *c, const int64_t *cells, int n, float inv_dl)
+{
+    if (!c || n < 0 || (n > 0 && !cells))
+        return fail(c, TLM_ERR_ARG, "tlm_set_probes: bad arguments");
+    if (c->nsteps <= 0)
+        return fail(c, TLM_ERR_STATE, "tlm_set_probes before tlm_set_schedule");
+    std::vector<long long> pc(n);
+    for (int i = 0; i < n; ++i)
+    {
+        if (!cell_ok(c, cells[i]))
+            return fail(c, TLM_ERR_ARG, "probe cell out of range");
+        pc[i] = padded_cell(c, cells[i]);
+    }
+    int rc;
+    if ((rc = upload(c, &c->prcell, pc.data(), pc.size())))
+        return rc;
+    c->npr = n;
+    c->inv_dl = inv_dl;
+    dfree(c, c->prrec);
+    if ((rc = dalloc(c, &c->prrec, (size_t)c->nsteps * std::max(1, n) * 3)))
+        return rc;
+    HIPCHK(c, hipMemset(c->prrec, 0, (size_t)c->nsteps * std::max(1, n) * 3 * sizeof(float)));
+    c->nfreq = 0;
+    return TLM_OK;
+}
+
+int tlm_set_probe_dft(tlm_ctx *c, const double *ctab, const double *stab, int nfreq)
+{
+    if (!c || nfreq < 0 || (nfreq > 0 && (!ctab || !stab)))
+        return fail(c, TLM_ERR_ARG, "tlm_set_probe_dft: bad arguments");
+    if (c->nsteps <= 0)
+        return fail(c, TLM_ERR_STATE, "tlm_set_probe_dft before tlm_set_schedule");
+    int rc;
+    const size_t tab = (size_t)c->nsteps * nfreq;
+    if ((rc = upload(c, &c->ctab, ctab, tab)) || (rc = upload(c, &c->stab, stab, tab)))
+        return rc;
+    const size_t acc = (size_t)std::max(1, c->npr) * 3 * std::max(1, nfreq);
+    dfree(c, c->dft_re);
+    dfree(c, c->dft_im);
+    if ((rc = dalloc(c, &c->dft_re, acc)) || (rc = dalloc(c, &c->dft_im, acc)))
+        return rc;
+    HIPCHK(c, hipMemset(c->dft_re, 0, acc * sizeof(double)));
+    HIPCHK(c, hipMemset(c->dft_im, 0, acc * sizeof(double)));
+    c->nfreq = nfreq;
+    c->dft_done = 0;
+    return TLM_OK;
+}
+
+int tlm_set_ports(tlm_ctx *c, const int64_t *cells, const int32_t *off, const int32_t *pol, int np)
+{
+    if (!c || np < 0 || (np > 0 && (!off || !pol)))
+        return fail(c, TLM_ERR_ARG, "tlm_set_ports: bad arguments");
+    if (c->nsteps <= 0)
+        return fail(c, TLM_ERR_STATE, "tlm_set_ports before tlm_set_schedule");
+    std::vector<long long> pc;
+    std::vector<long long> loop(4 * (size_t)std::max(1, np), 0);
+    std::vector<int> po(np + 1), pp(np);
+    const long long stride[3] = {1, c->nx, (long long)c->nx * c->ny};
+    for (int p = 0; p < np; ++p)
+    {
+        if (pol[p] < 0 || pol[p] > 2 || off[p + 1] < off[p])
+            return fail(c, TLM_ERR_ARG, "bad port description");
+        po[p] = (int)pc.size();
+        pp[p] = pol[p];
+        for (int s = off[p]; s < off[p + 1]; ++s)
+        {
+            if (!cell_ok(c, cells[s]))
+                return fail(c, TLM_ERR_ARG, "port cell out of range");
+            pc.push_back(padded_cell(c, cells[s]));
+        }
+        const int cnt = off[p + 1] - off[p];
+        if (cnt > 0)
+        {
+            // Ampere loop neighbours by linear index, as TlmSolver.cpp:687-698
+            long long mid = cells[off[p] + cnt / 2];
+            int t1 = (pol[p] + 1) % 3, t2 = (pol[p] + 2) % 3;
+            long long nb[4] = {mid + stride[t1], mid - stride[t1], mid + stride[t2], mid - stride[t2]};
+            for (int q = 0; q < 4; ++q)
+            {
+                if (!cell_ok(c, nb[q]))
+                    return fail(c, TLM_ERR_ARG, "port current loop leaves the grid");
+                loop[4 * p + q] = padded_cell(c, nb[q]);
+            }
+        }
+    }
+    po[np] = (int)pc.size();
+    int rc;
+    if ((rc = upload(c, &c->pcells, pc.data(), pc.size())) ||
+        (rc = upload(c, &c->poff, po.data(), po.size())) ||
+        (rc = upload(c, &c->ppol, pp.data(), pp.size())) ||
+        (rc = upload(c, &c->ploop, loop.data(), loop.size())))
+        return rc;
+    c->np = np;
+    dfree(c, c->prec);
+    if ((rc = dalloc(c, &c->prec, (size_t)c->nsteps * std::max(1, np) * 2)))
+        return rc;
+    HIPCHK(c, hipMemset(c->prec, 0, (size_t)c->nsteps * std::max(1, np) * 2 * sizeof(float)));
+    return TLM_OK;
+}
+
+int tlm_set_faces(tlm_ctx *c, const int64_t *cells, const int32_t *axes, int n)
+{
+    if (!c || n < 0 || (n > 0 && (!cells || !axes)))
+        return fail(c, TLM_ERR_ARG, "tlm_set_faces: bad arguments");
+    std::vector<long long> fc(n);
+    std::vector<int> fa(n);
+    for (int i = 0; i < n; ++i)
+    {
+        if (!cell_ok(c, cells[i]) || axes[i] < 0 || axes[i] > 2)
+            return fail(c, TLM_ERR_ARG, "face cell or axis out of range");
+        fc[i] = padded_cell(c, cells[i]);
+        fa[i] = axes[i];
+    }
+    int rc;
+    if ((rc = upload(c, &c->fcell, fc.data(), fc.size())) ||
+        (rc = upload(c, &c->faxis, fa.data(), fa.size())))
+        return rc;
+    dfree(c, c->js_inst);
+    dfree(c, c->js_dft4);
+    if ((rc = dalloc(c, &c->js_inst, std::max(1, n))) ||
+        (rc = dalloc(c, &c->js_dft4, 4 * (size_t)std::max(1, n))))
+        return rc;
+    HIPCHK(c, hipMemset(c->js_inst, 0, std::max(1, n) * sizeof(float)));
+    HIPCHK(c, hipMemset(c->js_dft4, 0, 4 * (size_t)std::max(1, n) * sizeof(float)));
+    c->nf = n;
+    return TLM_OK;
+}
+
+int tlm_set_huygens(tlm_ctx *c, const int64_t *cells, const int32_t *axes, int n)
+{
+    if (!c || n < 0 || (n > 0 && (!cells || !axes)))
+        return fail(c, TLM_ERR_ARG, "tlm_set_huygens: bad arguments");
+    std::vector<long long> hc(n);
+    std::vector<int> ha(n);
+    for (int i = 0; i < n; ++i)
+    {
+        if (!cell_ok(c, cells[i]) || axes[i] < 0 || axes[i] > 2)
+            return fail(c, TLM_ERR_ARG, "Huygens cell or axis out of range");
+        hc[i] = padded_cell(c, cells[i]);
+        ha[i] = axes[i];
+    }
+    int rc;
+    if ((rc = upload(c, &c->hcell, hc.data(), hc.size())) ||
+        (rc = upload(c, &c->haxis, ha.data(), ha.size())))
+        return rc;
+    dfree(c, c->huy8);
+    if ((rc = dalloc(c, &c->huy8, 8 * (size_t)std::max(1, n))))
+        return rc;
+    HIPCHK(c, hipMemset(c->huy8, 0, 8 * (size_t)std::max(1, n) * sizeof(float)));
+    c->nh = n;
+    return TLM_OK;
+}
+
+int tlm_run(tlm_ctx *c, int step0, int nsteps)
+{
+    if (!c)
+        return TLM_ERR_ARG;
+    return queue_steps(c, step0, nsteps, false, nullptr);
+}
+
+int tlm_sync(tlm_ctx *c)
+{
+    if (!c)
+        return TLM_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipGetLastError());
+    return TLM_OK;
+}
+
+int tlm_time_steps(tlm_ctx *c, int step0, int nsteps, float *ms_total, float *ms_kernel)
+{
+    if (!c)
+        return TLM_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    int rc = queue_steps(c, step0, nsteps, ms_kernel != nullptr, ms_kernel);
+    if (rc)
+        return rc;
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (ms_total)
+        *ms_total = ms;
+    return TLM_OK;
+}
+
+int tlm_launch_shape(const tlm_ctx *c, int *rows, int *planes, int *groups, int *threads)
+{
+    if (!c)
+        return TLM_ERR_ARG;
+    if (rows)
+        *rows = c->R;
+    if (planes)
+        *planes = c->zc;
+    if (groups)
+        *groups = c->gx * c->gy;
+    if (threads)
+        *threads = c->threads;
+    return TLM_OK;
+}
+
+int tlm_read_state(tlm_ctx *c, float *V, float *Vs)
+{
+    if (!c || !V)
+        return fail(c, TLM_ERR_ARG, "tlm_read_state: bad arguments");
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int rc = convert(c, nullptr, V, c->A, 12);
+    if (rc)
+        return rc;
+    if (Vs)
+    {
+        if (c->stubs)
+            return convert(c, nullptr, Vs, c->Vs, 3);
+        std::memset(Vs, 0, sizeof(float) * 3 * (size_t)c->nx * c->ny * c->nz);
+    }
+    return TLM_OK;
+}
+
+int tlm_read_probes(tlm_ctx *c, int step0, int nsteps, float *e)
+{
+    if (!c || step0 < 0 || nsteps < 0 || step0 + nsteps > c->nsteps || (!e && nsteps))
+        return fail(c, TLM_ERR_ARG, "tlm_read_probes: bad arguments");
+    if (c->npr == 0 || nsteps == 0)
+        return TLM_OK;
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(e, c->prrec + (size_t)step0 * c->npr * 3,
+                        (size_t)nsteps * c->npr * 3 * sizeof(float), hipMemcpyDeviceToHost));
+    return TLM_OK;
+}
+
+int tlm_read_probe_dft(tlm_ctx *c, double *re, double *im)
+{
+    if (!c || !re || !im)
+        return fail(c, TLM_ERR_ARG, "tlm_read_probe_dft: bad arguments");
+    if (c->npr == 0 || c->nfreq == 0)
+        return TLM_OK;
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const size_t n = (size_t)c->npr * 3 * c->nfreq;
+    HIPCHK(c, hipMemcpy(re, c->dft_re, n * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(im, c->dft_im, n * sizeof(double), hipMemcpyDeviceToHost));
+    return TLM_OK;
+}
+
+int tlm_read_ports(tlm_ctx *c, int step0, int nsteps, float *vi)
+{
+    if (!c || step0 < 0 || nsteps < 0 || step0 + nsteps > c->nsteps || (!vi && nsteps))
+        return fail(c, TLM_ERR_ARG, "tlm_read_ports: bad arguments");
+    if (c->np == 0 || nsteps == 0)
+        return TLM_OK;
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(vi, c->prec + (size_t)step0 * c->np * 2,
+                        (size_t)nsteps * c->np * 2 * sizeof(float), hipMemcpyDeviceToHost));
+    return TLM_OK;
+}
+
+int tlm_read_js(tlm_ctx *c, float *inst, float *dft4)
+{
+    if (!c)
+        return TLM_ERR_ARG;
+    if (c->nf == 0)
+        return TLM_OK;
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (inst)
+        HIPCHK(c, hipMemcpy(inst, c->js_inst, (size_t)c->nf * sizeof(float), hipMemcpyDeviceToHost));
+    if (dft4)
+        HIPCHK(c, hipMemcpy(dft4, c->js_dft4, (size_t)c->nf * 4 * sizeof(float),
+                            hipMemcpyDeviceToHost));
+    return TLM_OK;
+}
+
+int tlm_read_huygens(tlm_ctx *c, float *dft8)
+{
+    if (!c || !dft8)
+        return fail(c, TLM_ERR_ARG, "tlm_read_huygens: bad arguments");
+    if (c->nh == 0)
+        return TLM_OK;
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(dft8, c->huy8, (size_t)c->nh * 8 * sizeof(float), hipMemcpyDeviceToHost));
+    return TLM_OK;
+}
+
+int tlm_read_energy(tlm_ctx *c, int32_t *steps, double *vals, int cap)
+{
+    if (!c || cap < 0)
+        return fail(c, TLM_ERR_ARG, "tlm_read_energy: bad arguments");
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<int> idx;
+    for (size_t i = 0; i < c->energy_steps.size(); ++i)
+        if (c->energy_steps[i] >= 0)
+            idx.push_back((int)i);
+    const int cnt = (int)idx.size();
+    if (cap > 0 && (steps || vals))
+    {
+        std::vector<double> all(c->energy_steps.size());
+        if (!all.empty())
+            HIPCHK(c, hipMemcpy(all.data(), c->energy, all.size() * sizeof(double),
+                                hipMemcpyDeviceToHost));
+        for (int q = 0; q < std::min(cap, cnt); ++q)
+        {
+            if (steps)
+                steps[q] = c->energy_steps[idx[q]];
+            if (vals)
+                vals[q] = all[idx[q]];
+        }
+    }
+    return cnt;
+}
+
+float tlm_scatter_unitarity_error(float epsr)
+{
+    if (tlm_device_count() <= 0)
+        return (float)TLM_ERR_NO_DEVICE;
+    const bool diel = epsr > 1.0f;
+    const int NP = diel ? 15 : 12;
+    float yh = diel ? 4.0f * (epsr - 1.0f) : 0.0f;
+    float den = 4.0f + yh + 0.0f;
+    float *dS = nullptr;
+    if (hipMalloc((void **)&dS, 15 * 15 * sizeof(float)) != hipSuccess)
+        return (float)TLM_ERR_NOMEM;
+    hipLaunchKernelGGL(tlm::k_scatter_columns, dim3(1), dim3(64), 0, 0, yh, den, diel ? 1 : 0, dS);
+    float S[15 * 15];
+    hipError_t e = hipMemcpy(S, dS, sizeof(S), hipMemcpyDeviceToHost);
+    (void)hipFree(dS);
+    if (e != hipSuccess)
+        return (float)TLM_ERR_HIP;
+    // S^T W S against W with W = (1 x 12, yh x 3), as TlmSolver.cpp:141-169
+    float W[15];
+    for (int i = 0; i < 15; ++i)
+        W[i] = i < 12 ? 1.0f : yh;
+    float maxErr = 0.0f;
+    for (int a = 0; a < NP; ++a)
+        for (int b = 0; b < NP; ++b)
+        {
+            float dot = 0.0f;
+            for (int r = 0; r < NP; ++r)
+                dot += W[r] * S[r * 15 + a] * S[r * 15 + b];
+            float want = (a == b) ? W[a] : 0.0f;
+            maxErr = std::max(maxErr, std::fabs(dot - want));
+        }
+    return maxErr;
+}
+
+int64_t tlm_build_pec_faces(int nx, int ny, int nz, const uint8_t *mat, int64_t *cells,
+                            int32_t *axes, int32_t *signs, int64_t cap)
+{
+    if (!mat || nx <= 0 || ny <= 0 || nz <= 0)
+        return 0;
+    const long long dirs[3] = {1, nx, (long long)nx * ny};
+    const int nn[3] = {nx, ny, nz};
+    int64_t cnt = 0;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i)
+            {
+                long long c = ((long long)k * ny + j) * nx + i;
+                if (mat[c] != 1)
+                    continue;
+                const int co[3] = {i, j, k};
+                for (int a = 0; a < 3; ++a)
+                    for (int s = -1; s <= 1; s += 2)
+                    {
+                        int qq = co[a] + s;
+                        if (qq < 0 || qq >= nn[a])
+                            continue;
+                        long long nb = c + s * dirs[a];
+                        if (mat[nb] == 1)
+                            continue;
+                        if (cnt < cap)
+                        {
+                            if (cells)
+                                cells[cnt] = nb;
+                            if (axes)
+                                axes[cnt] = a;
+                            if (signs)
+                                signs[cnt] = s;
+                        }
+                        ++cnt;
+                    }
+            }
+    return cnt;
+}
+
+int64_t tlm_build_huygens(int nx, int ny, int nz, const uint8_t *mat, int o, int64_t *cells,
+                          int32_t *axes, int32_t *out_signs, int64_t cap)
+{
+    if (!mat || o < 2)
+        return 0;
+    const int nn[3] = {nx, ny, nz};
+    if (2 * o + 2 >= nn[0] || 2 * o + 2 >= nn[1] || 2 * o + 2 >= nn[2])
+        return 0;
+    int64_t cnt = 0;
+    auto add = [&](int i, int j, int k, int axis, int sg) {
+        long long c = ((long long)k * ny + j) * nx + i;
+        if (mat[c] != 0)
+            return;
+        if (cnt < cap)
+        {
+            if (cells)
+                cells[cnt] = c;
+            if (axes)
+                axes[cnt] = axis;
+            if (out_signs)
+                out_signs[cnt] = sg;
+        }
+        ++cnt;
+    };
+    int x0 = o, x1 = nx - 1 - o, y0 = o, y1 = ny - 1 - o, z0 = o, z1 = nz - 1 - o;
+    for (int k = z0; k <= z1; ++k)
+        for (int j = y0; j <= y1; ++j)
+        {
+            add(x0, j, k, 0, -1);
+            add(x1, j, k, 0, +1);
+        }
+    for (int k = z0; k <= z1; ++k)
+        for (int i = x0; i <= x1; ++i)
+        {
+            add(i, y0, k, 1, -1);
+            add(i, y1, k, 1, +1);
+        }
+    for (int j = y0; j <= y1; ++j)
+        for (int i = x0; i <= x1; ++i)
+        {
+            add(i, j, z0, 2, -1);
+            add(i, j, z1, 2, +1);
+        }
+    return cnt;
+}
+
+void tlm_schedule_tables(int type, float f0, float dt, int nsteps, float gdw, float gdd, float *sv,
+                         float *cph, float *sph)
+{
+    for (int n = 0; n < nsteps; ++n)
+    {
+        if (sv)
+        {
+            float val;
+            if (type == TLM_WF_GAUSS_DERIV)
+            {
+                double u = ((double)n - (double)gdd) / (double)gdw;
+                val = (float)(-std::sqrt(2.0 * std::exp(1.0)) * u * std::exp(-u * u));
+            }
+            else
+            {
+                float t = n * dt;
+                float w = 2.0f * (float)M_PI * f0;
+                if (type == TLM_WF_CW_RAMPED)
+                {
+                    float Tr = 3.0f / f0;
+                    float r = (t < Tr) ? 0.5f * (1.0f - std::cos((float)M_PI * t / Tr)) : 1.0f;
+                    val = r * std::sin(w * t);
+                }
+                else if (type == TLM_WF_GAUSS_SINE)
+                {
+                    float tau = 2.0f / f0, t0 = 3.0f * tau;
+                    float u = (t - t0) / tau;
+                    val = std::exp(-u * u) * std::sin(w * (t - t0));
+                }
+                else
+                {
+                    float tau = 0.5f / f0, t0 = 3.0f * tau;
+                    float u = (t - t0) / tau;
+                    val = std::exp(-u * u);
+                }
+            }
+            sv[n] = val;
+        }
+        float phase = 2.0f * (float)M_PI * f0 * (n * dt);
+        if (cph)
+            cph[n] = std::cos(phase);
+        if (sph)
+            sph[n] = std::sin(phase);
+    }
+}
+
+void tlm_dft_tables(const double *freqs, int nfreq, double dt, int nsteps, double *ctab,
+                    double *stab)
+{
+    for (int n = 0; n < nsteps; ++n)
+        for (int q = 0; q < nfreq; ++q)
+        {
+            double ph = 2.0 * M_PI * freqs[q] * ((double)n * dt);
+            ctab[(size_t)n * nfreq + q] = std::cos(ph);
+            stab[(size_t)n * nfreq + q] = std::sin(ph);
+        }
+}
+
+} // extern "C"

@@ -1,0 +1,389 @@
+"""Python handle on the gfx950 TLM step (ctypes over include/GpuTlmHip.h).
+
+``TlmDevice`` wraps one ``tlm_ctx``: upload, per-step tables, monitors, run,
+readback. ``run_scene`` drives a whole :class:`emsolver_amd.scenes.Scene`
+the way ``TlmSolver::run`` drives ``RunGpuTlm`` (TlmSolver.cpp:439-454): it
+builds the injection list (sources, then the AIR cells of every port,
+TlmSolver.cpp:362-374 and 501-505), the PEC face list, the per-step
+waveform and phase tables, runs every step on the device and reads the
+monitors back.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+
+import numpy as np
+
+from . import _native
+from ._native import ptr
+
+C0 = np.float32(299792458.0)      # TlmSolver.cpp:15
+ETA0 = np.float32(376.730313)     # TlmSolver.cpp:16
+
+PULSE_TYPES = {"f32": 0, "f16": 1, "bf16": 2}
+
+
+class TlmError(RuntimeError):
+    pass
+
+
+def device_count() -> int:
+    return _native.load().tlm_device_count()
+
+
+def device_name(dev: int = 0) -> str:
+    return _native.load().tlm_device_name(dev).decode()
+
+
+def scatter_unitarity_error(epsr: float = 1.0) -> float:
+    """ScatterUnitarityError / DielScatterUnitarityError on the device scatter."""
+    return float(_native.load().tlm_scatter_unitarity_error(float(epsr)))
+
+
+def timestep(dl: float) -> np.float32:
+    """dt = dl / (2 c0) in float, TlmSolver.cpp:268."""
+    return np.float32(dl) / (np.float32(2.0) * C0)
+
+
+def stub_admittances(epsr, sigma, dl):
+    """yhat = 4(max(1,epsr)-1), ghat = max(0,sigma)*dl*eta0 in float,
+    TlmSolver.cpp:272-276."""
+    epsr = np.asarray(epsr, np.float32)
+    sigma = np.asarray(sigma, np.float32)
+    yh = np.float32(4.0) * (np.maximum(np.float32(1.0), epsr) - np.float32(1.0))
+    gh = (np.maximum(np.float32(0.0), sigma) * np.float32(dl)) * ETA0
+    return yh.astype(np.float32), gh.astype(np.float32)
+
+
+def schedule_tables(waveform: int, f0: float, dt: float, nsteps: int, gd_width=20.0,
+                    gd_delay=60.0):
+    lib = _native.load()
+    sv = np.empty(nsteps, np.float32)
+    cph = np.empty(nsteps, np.float32)
+    sph = np.empty(nsteps, np.float32)
+    lib.tlm_schedule_tables(waveform, f0, float(dt), nsteps, gd_width, gd_delay,
+                            ptr(sv, C.c_float), ptr(cph, C.c_float), ptr(sph, C.c_float))
+    return sv, cph, sph
+
+
+def dft_tables(freqs, dt: float, nsteps: int):
+    lib = _native.load()
+    f = np.ascontiguousarray(freqs, np.float64)
+    ct = np.empty((nsteps, f.size))
+    st = np.empty((nsteps, f.size))
+    lib.tlm_dft_tables(ptr(f, C.c_double), f.size, float(dt), nsteps, ptr(ct, C.c_double),
+                       ptr(st, C.c_double))
+    return ct, st
+
+
+def pec_faces(nx, ny, nz, mat):
+    """BuildPecSurfaceFaces order (Geometry.cpp:402-436): air cell, axis, sign."""
+    lib = _native.load()
+    mat = np.ascontiguousarray(mat, np.uint8).reshape(-1)
+    n = lib.tlm_build_pec_faces(nx, ny, nz, ptr(mat, C.c_uint8), None, None, None, 0)
+    cells = np.empty(n, np.int64)
+    axes = np.empty(n, np.int32)
+    signs = np.empty(n, np.int32)
+    lib.tlm_build_pec_faces(nx, ny, nz, ptr(mat, C.c_uint8), ptr(cells, C.c_int64),
+                            ptr(axes, C.c_int32), ptr(signs, C.c_int32), n)
+    return cells, axes, signs
+
+
+def huygens_faces(nx, ny, nz, mat, offset):
+    lib = _native.load()
+    mat = np.ascontiguousarray(mat, np.uint8).reshape(-1)
+    n = lib.tlm_build_huygens(nx, ny, nz, ptr(mat, C.c_uint8), offset, None, None, None, 0)
+    cells = np.empty(n, np.int64)
+    axes = np.empty(n, np.int32)
+    sg = np.empty(n, np.int32)
+    lib.tlm_build_huygens(nx, ny, nz, ptr(mat, C.c_uint8), offset, ptr(cells, C.c_int64),
+                          ptr(axes, C.c_int32), ptr(sg, C.c_int32), n)
+    return cells, axes, sg
+
+
+class TlmDevice:
+    """One device context (``tlm_ctx``)."""
+
+    def __init__(self, nx, ny, nz, dl, rho=0.0, pulse="f32", device=0):
+        self.lib = _native.load()
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.n = self.nx * self.ny * self.nz
+        d = _native.GridDesc(self.nx, self.ny, self.nz, float(dl), float(rho),
+                             PULSE_TYPES[pulse], int(device), 0)
+        h = C.c_void_p()
+        rc = self.lib.tlm_create(C.byref(d), C.byref(h))
+        if rc != 0:
+            raise TlmError(f"tlm_create failed ({rc}): {self.lib.tlm_last_error(None).decode()}")
+        self.h = h
+        self.nprobe = 0
+        self.nport = 0
+        self.nface = 0
+        self.nhuy = 0
+        self.nfreq = 0
+        self.has_stubs = False
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise TlmError(f"{what} failed ({rc}): {self.lib.tlm_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.tlm_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- set-up ----
+    def upload(self, mat, yhat=(), ghat=(), V=None, Vs=None):
+        mat = np.ascontiguousarray(mat, np.uint8).reshape(-1)
+        if mat.size != self.n:
+            raise TlmError("material map must hold nx*ny*nz bytes")
+        yh = np.ascontiguousarray(yhat, np.float32).reshape(-1)
+        gh = np.ascontiguousarray(ghat, np.float32).reshape(-1)
+        Va = None if V is None else np.ascontiguousarray(V, np.float32).reshape(-1)
+        Vsa = None if Vs is None else np.ascontiguousarray(Vs, np.float32).reshape(-1)
+        if Va is not None and Va.size != 12 * self.n:
+            raise TlmError("V must hold 12 floats per cell")
+        if Vsa is not None and Vsa.size != 3 * self.n:
+            raise TlmError("Vs must hold 3 floats per cell")
+        self._chk(self.lib.tlm_upload(self.h, ptr(mat, C.c_uint8), ptr(yh, C.c_float),
+                                      ptr(gh, C.c_float), yh.size, ptr(Va, C.c_float),
+                                      ptr(Vsa, C.c_float)), "tlm_upload")
+        self.has_stubs = bool((mat >= 2).any())
+
+    def set_sources(self, cells, pols, amps):
+        c = np.ascontiguousarray(cells, np.int64)
+        p = np.ascontiguousarray(pols, np.int32)
+        a = np.ascontiguousarray(amps, np.float32)
+        self._chk(self.lib.tlm_set_sources(self.h, ptr(c, C.c_int64), ptr(p, C.c_int32),
+                                           ptr(a, C.c_float), c.size), "tlm_set_sources")
+
+    def set_schedule(self, sv, cph, sph, settle):
+        sv = np.ascontiguousarray(sv, np.float32)
+        cph = np.ascontiguousarray(cph, np.float32)
+        sph = np.ascontiguousarray(sph, np.float32)
+        self._chk(self.lib.tlm_set_schedule(self.h, ptr(sv, C.c_float), ptr(cph, C.c_float),
+                                            ptr(sph, C.c_float), sv.size, int(settle)),
+                  "tlm_set_schedule")
+        self.nsteps = sv.size
+
+    def set_probes(self, cells, inv_dl):
+        c = np.ascontiguousarray(cells, np.int64)
+        self._chk(self.lib.tlm_set_probes(self.h, ptr(c, C.c_int64), c.size, float(inv_dl)),
+                  "tlm_set_probes")
+        self.nprobe = c.size
+
+    def set_probe_dft(self, ctab, stab):
+        ct = np.ascontiguousarray(ctab, np.float64)
+        st = np.ascontiguousarray(stab, np.float64)
+        nf = ct.shape[1] if ct.ndim == 2 else 0
+        self._chk(self.lib.tlm_set_probe_dft(self.h, ptr(ct, C.c_double), ptr(st, C.c_double),
+                                             nf), "tlm_set_probe_dft")
+        self.nfreq = nf
+
+    def set_ports(self, ports):
+        """ports: list of (cells, pol_axis)."""
+        cells = [np.asarray(c, np.int64).reshape(-1) for c, _ in ports]
+        off = np.zeros(len(ports) + 1, np.int32)
+        for i, c in enumerate(cells):
+            off[i + 1] = off[i] + c.size
+        flat = np.ascontiguousarray(np.concatenate(cells) if cells else np.zeros(0, np.int64))
+        pol = np.ascontiguousarray([p for _, p in ports], np.int32)
+        self._chk(self.lib.tlm_set_ports(self.h, ptr(flat, C.c_int64), ptr(off, C.c_int32),
+                                         ptr(pol, C.c_int32), len(ports)), "tlm_set_ports")
+        self.nport = len(ports)
+
+    def set_faces(self, cells, axes):
+        c = np.ascontiguousarray(cells, np.int64)
+        a = np.ascontiguousarray(axes, np.int32)
+        self._chk(self.lib.tlm_set_faces(self.h, ptr(c, C.c_int64), ptr(a, C.c_int32), c.size),
+                  "tlm_set_faces")
+        self.nface = c.size
+
+    def set_huygens(self, cells, axes):
+        c = np.ascontiguousarray(cells, np.int64)
+        a = np.ascontiguousarray(axes, np.int32)
+        self._chk(self.lib.tlm_set_huygens(self.h, ptr(c, C.c_int64), ptr(a, C.c_int32), c.size),
+                  "tlm_set_huygens")
+        self.nhuy = c.size
+
+    def set_energy_every(self, every):
+        self._chk(self.lib.tlm_set_energy_every(self.h, int(every)), "tlm_set_energy_every")
+
+    # ---- execution ----
+    def run(self, step0, nsteps):
+        self._chk(self.lib.tlm_run(self.h, int(step0), int(nsteps)), "tlm_run")
+
+    def sync(self):
+        self._chk(self.lib.tlm_sync(self.h), "tlm_sync")
+
+    def time_steps(self, step0, nsteps, kernel=True):
+        tot = C.c_float(0)
+        ker = C.c_float(0)
+        self._chk(self.lib.tlm_time_steps(self.h, int(step0), int(nsteps), C.byref(tot),
+                                          C.byref(ker) if kernel else None), "tlm_time_steps")
+        return tot.value, (ker.value if kernel else None)
+
+    def launch_shape(self):
+        r, z, g, t = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        self._chk(self.lib.tlm_launch_shape(self.h, C.byref(r), C.byref(z), C.byref(g),
+                                            C.byref(t)), "tlm_launch_shape")
+        return {"rows_per_group": r.value, "planes_per_group": z.value, "groups": g.value,
+                "threads_per_group": t.value}
+
+    # ---- readback ----
+    def read_state(self):
+        V = np.empty(12 * self.n, np.float32)
+        Vs = np.empty(3 * self.n, np.float32)
+        self._chk(self.lib.tlm_read_state(self.h, ptr(V, C.c_float), ptr(Vs, C.c_float)),
+                  "tlm_read_state")
+        return V, Vs
+
+    def read_probes(self, step0, nsteps):
+        e = np.empty((nsteps, self.nprobe, 3), np.float32)
+        self._chk(self.lib.tlm_read_probes(self.h, step0, nsteps, ptr(e, C.c_float)),
+                  "tlm_read_probes")
+        return e
+
+    def read_probe_dft(self):
+        re = np.empty((self.nprobe, 3, self.nfreq))
+        im = np.empty((self.nprobe, 3, self.nfreq))
+        self._chk(self.lib.tlm_read_probe_dft(self.h, ptr(re, C.c_double), ptr(im, C.c_double)),
+                  "tlm_read_probe_dft")
+        return re, im
+
+    def read_ports(self, step0, nsteps):
+        vi = np.empty((nsteps, self.nport, 2), np.float32)
+        self._chk(self.lib.tlm_read_ports(self.h, step0, nsteps, ptr(vi, C.c_float)),
+                  "tlm_read_ports")
+        return vi
+
+    def read_js(self):
+        inst = np.empty(self.nface, np.float32)
+        d4 = np.empty((self.nface, 4), np.float32)
+        self._chk(self.lib.tlm_read_js(self.h, ptr(inst, C.c_float), ptr(d4, C.c_float)),
+                  "tlm_read_js")
+        return inst, d4
+
+    def read_huygens(self):
+        d8 = np.empty((self.nhuy, 8), np.float32)
+        self._chk(self.lib.tlm_read_huygens(self.h, ptr(d8, C.c_float)), "tlm_read_huygens")
+        return d8
+
+    def read_energy(self):
+        n = self.lib.tlm_read_energy(self.h, None, None, 0)
+        if n < 0:
+            self._chk(n, "tlm_read_energy")
+        st = np.empty(n, np.int32)
+        v = np.empty(n, np.float64)
+        self.lib.tlm_read_energy(self.h, ptr(st, C.c_int32), ptr(v, C.c_double), n)
+        return st, v
+
+
+@dataclasses.dataclass
+class SceneResult:
+    steps: int
+    probes: np.ndarray | None = None      # [steps][nprobe][3]
+    probe_dft: tuple | None = None        # (re, im) [nprobe][3][nfreq]
+    ports: np.ndarray | None = None       # [steps][nport][2]
+    js_inst: np.ndarray | None = None
+    js_dft4: np.ndarray | None = None
+    dft_samples: int = 0
+    huygens: np.ndarray | None = None
+    huy_samples: int = 0
+    energy_steps: np.ndarray | None = None
+    energy: np.ndarray | None = None
+    V: np.ndarray | None = None
+    Vs: np.ndarray | None = None
+    kernel_ms: float | None = None
+    total_ms: float | None = None
+
+
+def open_scene(scene, device=0) -> TlmDevice:
+    """Create a context and load a Scene into it (no steps run)."""
+    dev = TlmDevice(scene.nx, scene.ny, scene.nz, scene.dl, scene.rho, device=device)
+    yh, gh = stub_admittances(scene.epsr, scene.sigma, scene.dl)
+    dev.upload(scene.mat, yh, gh, scene.V0, scene.Vs0)
+    dt = timestep(scene.dl)
+    total = scene.total_steps
+    sv, cph, sph = schedule_tables(scene.waveform, scene.f0, dt, total, scene.gd_width,
+                                   scene.gd_delay)
+    dev.set_schedule(sv, cph, sph, scene.settle_steps)
+    dev.set_energy_every(scene.energy_every)
+    # injection order: sources, then the AIR cells of each port (addPort filter)
+    mat = scene.mat.reshape(-1)
+    cells, pols, amps = [], [], []
+    for c, p, a in scene.sources:
+        cells.append(c)
+        pols.append(p)
+        amps.append(a)
+    port_list = []
+    for pc, p, a in scene.ports:
+        pc = [int(c) for c in pc if 0 <= c < mat.size and mat[c] == 0]
+        port_list.append((pc, p))
+        for c in pc:
+            cells.append(c)
+            pols.append(p)
+            amps.append(a)
+    dev.set_sources(cells, pols, amps)
+    if port_list:
+        dev.set_ports(port_list)
+    if scene.faces:
+        fc, fa, _ = pec_faces(scene.nx, scene.ny, scene.nz, mat)
+        if fc.size:
+            dev.set_faces(fc, fa)
+    if scene.huy_offset >= 2:
+        hc, ha, _ = huygens_faces(scene.nx, scene.ny, scene.nz, mat, scene.huy_offset)
+        if hc.size:
+            dev.set_huygens(hc, ha)
+    if len(scene.probes):
+        dev.set_probes(scene.probes, np.float32(-1.0) / np.float32(scene.dl))
+        if len(scene.dft_freqs):
+            ct, st = dft_tables(scene.dft_freqs, float(dt), total)
+            dev.set_probe_dft(ct, st)
+    dev.dt = dt
+    return dev
+
+
+def collect(dev: TlmDevice, scene, steps: int, state=True) -> SceneResult:
+    res = SceneResult(steps=steps)
+    if dev.nprobe:
+        res.probes = dev.read_probes(0, steps)
+        if dev.nfreq:
+            res.probe_dft = dev.read_probe_dft()
+    if dev.nport:
+        res.ports = dev.read_ports(0, steps)
+    if dev.nface:
+        res.js_inst, res.js_dft4 = dev.read_js()
+    res.dft_samples = max(0, steps - scene.settle_steps)
+    if dev.nhuy:
+        res.huygens = dev.read_huygens()
+        res.huy_samples = res.dft_samples
+    res.energy_steps, res.energy = dev.read_energy()
+    if state:
+        res.V, res.Vs = dev.read_state()
+    return res
+
+
+def run_scene(scene, steps=None, device=0, state=True, block=256) -> SceneResult:
+    """Run a Scene on the GPU from step 0 and return every monitor."""
+    steps = scene.total_steps if steps is None else steps
+    with open_scene(scene, device) as dev:
+        n = 0
+        while n < steps:
+            m = min(block, steps - n)
+            dev.run(n, m)
+            n += m
+        dev.sync()
+        return collect(dev, scene, steps, state)

@@ -1,0 +1,201 @@
+/*
+ * GpuTlmHip.h - C ABI of the MI355X (gfx950) SCN TLM time step.
+ *
+ * This is the thin extern "C" layer that replaces the OpenGL compute-shader
+ * dispatch behind the reference's GPU entry point
+ *     bool RunGpuTlm(TlmSolver &s, std::string &msg);     (GpuTlm.h:15)
+ * The host glue emsolver_amd/host/GpuTlm.cpp implements RunGpuTlm on top of
+ * these functions; nothing here uses C++ or torch types.
+ *
+ * Conventions
+ *   - Return codes: TLM_OK (0) on success, a negative TLM_ERR_* otherwise;
+ *     tlm_last_error() gives the message (the reference's bool + msg).
+ *   - Cells are linear indices (k*ny + j)*nx + i (VoxelGridSpec::cellIndex,
+ *     Geometry.h:91-94). Host arrays are caller owned. Pulses cross the
+ *     boundary in the reference's array-of-structures order (12 floats per
+ *     cell, port = axis*4 + side*2 + pol, TlmSolver.cpp:20-29; 3 stub floats
+ *     per cell); the device keeps them as structure-of-arrays planes.
+ *   - One context is used by one host thread at a time; separate contexts
+ *     may be driven from separate threads. Work is queued on the context's
+ *     own HIP stream; tlm_sync() waits for it.
+ */
+#ifndef GPU_TLM_HIP_H
+#define GPU_TLM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    TLM_OK = 0,
+    TLM_ERR_ARG = -1,         /* bad argument / shape */
+    TLM_ERR_NO_DEVICE = -2,   /* no HIP device (RunGpuTlm returns false) */
+    TLM_ERR_NOMEM = -3,       /* device allocation failed (GpuTlm.cpp:664-668) */
+    TLM_ERR_HIP = -4,         /* other HIP runtime error */
+    TLM_ERR_STATE = -5,       /* call out of order (e.g. run before upload) */
+    TLM_ERR_UNSUPPORTED = -6  /* configuration this build does not handle */
+};
+
+/* Link-pulse storage type. Arithmetic is always FP32. */
+enum { TLM_PULSE_F32 = 0, TLM_PULSE_F16 = 1, TLM_PULSE_BF16 = 2 };
+
+typedef struct tlm_grid_desc {
+    int32_t nx, ny, nz;  /* VoxelGridSpec extents (Geometry.h:87) */
+    float dl;            /* cell size in metres */
+    float rho;           /* outer boundary reflection, TlmConfig::boundaryRho (TlmSolver.h:119) */
+    int32_t pulse_type;  /* TLM_PULSE_* */
+    int32_t device;      /* HIP device ordinal */
+    int32_t reserved;    /* must be 0 */
+} tlm_grid_desc;
+
+typedef struct tlm_ctx tlm_ctx;
+
+/* Number of visible HIP devices (0 without a GPU). */
+int tlm_device_count(void);
+
+/* Adapter name of a device; replaces GlRendererName() (GpuTlm.cpp:463-467). */
+const char *tlm_device_name(int device);
+
+/* Creates a context: picks the device, allocates the ping-pong pulse planes
+ * and the material map. Replaces the GL context/buffer set-up of
+ * GpuTlm::run (GpuTlm.cpp:533-668). On failure *out is NULL and the message
+ * is available from tlm_last_error(NULL). */
+int tlm_create(const tlm_grid_desc *desc, tlm_ctx **out);
+
+/* Frees every device allocation of the context (RAII GpuCtx, GpuTlm.cpp:384-407). */
+void tlm_destroy(tlm_ctx *ctx);
+
+/* Last error message of the context (or of the last failed tlm_create when
+ * ctx is NULL). Never NULL. */
+const char *tlm_last_error(const tlm_ctx *ctx);
+
+/* Uploads the material map, the per-dielectric stub admittances and the
+ * initial pulses (GpuTlm.cpp:564-585). mat has nx*ny*nz bytes: 0 air,
+ * 1 PEC, id >= 2 dielectric with yhat[id-2], ghat[id-2] (TlmSolver.cpp:
+ * 272-276). V_aos12 (12 floats per cell) and Vs_aos3 (3 per cell) may be
+ * NULL for zeros. Stubs exist iff a cell has id >= 2 (TlmSolver.cpp:277-283). */
+int tlm_upload(tlm_ctx *ctx, const uint8_t *mat, const float *yhat, const float *ghat, int nmat,
+               const float *V_aos12, const float *Vs_aos3);
+
+/* Soft sources in injection order: TlmSolver::sources, then every port's
+ * cells (TlmSolver.cpp:501-505; the GL source list GpuTlm.cpp:588-597).
+ * Each adds q = 0.5*amp*sv(n) to the 4 ports polarised along pol_axis. */
+int tlm_set_sources(tlm_ctx *ctx, const int64_t *cells, const int32_t *pol_axis, const float *amp,
+                    int n);
+
+/* Per-step host scalars for steps [0, nsteps): sv(n) = waveformValue(n)
+ * (TlmSolver.cpp:410-436) and the f0 phasor cph/sph (TlmSolver.cpp:619-620;
+ * GpuTlm.cpp:716-719). Steps n >= settle_steps accumulate the Js and
+ * Huygens DFTs. Allocates the per-step record buffers, so it is called
+ * before the monitor set-up functions below. */
+int tlm_set_schedule(tlm_ctx *ctx, const float *sv, const float *cph, const float *sph, int nsteps,
+                     int settle_steps);
+
+/* E-field probes, recorded every step after injection:
+ * E_j = inv_dl * VjOf(cell, j) with inv_dl = -1/dl (TlmSolver.cpp:705-713). */
+int tlm_set_probes(tlm_ctx *ctx, const int64_t *cells, int n, float inv_dl);
+
+/* Running FP64 DFT of every probe component: cos_tab/sin_tab hold
+ * [nsteps][nfreq] phasors; re += E*cos, im -= E*sin, summed in step order. */
+int tlm_set_probe_dft(tlm_ctx *ctx, const double *cos_tab, const double *sin_tab, int nfreq);
+
+/* Lumped ports: port p owns cells[cell_offsets[p] .. cell_offsets[p+1]).
+ * Records V = sequential float sum of VjOf over the cells and
+ * I = -2 * ZIloop / eta0 around the middle cell (TlmSolver.cpp:677-703). */
+int tlm_set_ports(tlm_ctx *ctx, const int64_t *cells, const int32_t *cell_offsets,
+                  const int32_t *pol_axis, int nports);
+
+/* PEC surface faces for the Js monitor, in BuildPecSurfaceFaces order
+ * (Geometry.cpp:402-436): air cell and normal axis (TlmSolver.cpp:627-649). */
+int tlm_set_faces(tlm_ctx *ctx, const int64_t *air_cells, const int32_t *axes, int n);
+
+/* Huygens box faces (TlmSolver.cpp:313-359, 651-675). */
+int tlm_set_huygens(tlm_ctx *ctx, const int64_t *cells, const int32_t *axes, int n);
+
+/* Energy sum(V^2) in FP64 every `every` steps (16 in TlmSolver.cpp:767);
+ * 0 disables. */
+int tlm_set_energy_every(tlm_ctx *ctx, int every);
+
+/* Queues steps [step0, step0 + nsteps): inject -> monitors -> fused
+ * scatter/connect (TlmSolver::step, TlmSolver.cpp:485-513). Asynchronous. */
+int tlm_run(tlm_ctx *ctx, int step0, int nsteps);
+
+/* Waits for the queued work; reports asynchronous HIP errors. */
+int tlm_sync(tlm_ctx *ctx);
+
+/* Current pulses back in reference AoS order (for snapshot parity; the GL
+ * path never read V back). Vs_aos3 may be NULL. Synchronous. */
+int tlm_read_state(tlm_ctx *ctx, float *V_aos12, float *Vs_aos3);
+
+/* Probe records of steps [step0, step0+nsteps): [nsteps][nprobes][3]. */
+int tlm_read_probes(tlm_ctx *ctx, int step0, int nsteps, float *e);
+
+/* FP64 probe DFT accumulators: [nprobes][3][nfreq] each. */
+int tlm_read_probe_dft(tlm_ctx *ctx, double *re, double *im);
+
+/* Port records of steps [step0, step0+nsteps): [nsteps][nports][2] (V, I),
+ * the GL portRec layout (GpuTlm.cpp:321-322). */
+int tlm_read_ports(tlm_ctx *ctx, int step0, int nsteps, float *vi);
+
+/* Js monitor: instantaneous |Js| per face and the float DFT accumulators
+ * [nfaces][4] = re1, im1, re2, im2 (GpuTlm.cpp:242-248). Either may be NULL. */
+int tlm_read_js(tlm_ctx *ctx, float *inst, float *dft4);
+
+/* Huygens accumulators, [nfaces][8] (TlmSolver.cpp:667-671). */
+int tlm_read_huygens(tlm_ctx *ctx, float *dft8);
+
+/* Energy samples recorded so far (steps and FP64 sums); returns the count,
+ * writing at most cap entries. */
+int tlm_read_energy(tlm_ctx *ctx, int32_t *steps, double *vals, int cap);
+
+/* Runs steps [step0, step0+nsteps) and measures them with HIP events on the
+ * context stream: *ms_total spans the whole queue of the steps, *ms_kernel
+ * is the summed duration of the fused scatter/connect launches alone. */
+int tlm_time_steps(tlm_ctx *ctx, int step0, int nsteps, float *ms_total, float *ms_kernel);
+
+/* Launch shape chosen for the fused kernel: rows per workgroup, planes per
+ * workgroup, workgroup count. For reporting. */
+int tlm_launch_shape(const tlm_ctx *ctx, int *rows_per_group, int *planes_per_group,
+                     int *groups, int *threads_per_group);
+
+/* ScatterUnitarityError / DielScatterUnitarityError (TlmSolver.cpp:116-169)
+ * evaluated with the device scatter: returns max |S^T W S - W| for the air
+ * node (epsr <= 1) or the stub node with yhat = 4(epsr-1), or a negative
+ * TLM_ERR_* code cast to float when no device is present. */
+float tlm_scatter_unitarity_error(float epsr);
+
+/* Host helpers shared by the glue code (no device needed). */
+
+/* BuildPecSurfaceFaces (Geometry.cpp:402-436): fills at most cap faces,
+ * returns the total count. air_cells/axes/signs may be NULL to count. */
+int64_t tlm_build_pec_faces(int nx, int ny, int nz, const uint8_t *mat, int64_t *air_cells,
+                            int32_t *axes, int32_t *signs, int64_t cap);
+
+/* TlmSolver::buildHuygens (TlmSolver.cpp:313-359): box inset `offset`
+ * cells, AIR cells only. Returns the count (0 when offset < 2). */
+int64_t tlm_build_huygens(int nx, int ny, int nz, const uint8_t *mat, int offset,
+                          int64_t *cells, int32_t *axes, int32_t *out_signs, int64_t cap);
+
+/* FP64 DFT phasor tables for tlm_set_probe_dft: ph = 2*pi*f*(n*dt),
+ * cos_tab[n*nfreq+q] = cos(ph), sin_tab[...] = sin(ph). */
+void tlm_dft_tables(const double *freqs, int nfreq, double dt, int nsteps, double *cos_tab,
+                    double *sin_tab);
+
+/* Waveform types of TlmConfig (TlmSolver.h:70) plus the Gaussian-derivative
+ * pulse sv = -sqrt(2e) u exp(-u^2), u = (n - delay)/width (in steps). */
+enum { TLM_WF_CW_RAMPED = 0, TLM_WF_GAUSS_SINE = 1, TLM_WF_GAUSS_PULSE = 2, TLM_WF_GAUSS_DERIV = 3 };
+
+/* Per-step host scalars for tlm_set_schedule: sv(n) = waveformValue(n)
+ * (TlmSolver.cpp:410-436) and cph/sph of the f0 phase 2*pi*f0*(n*dt)
+ * (TlmSolver.cpp:619-620), all in float like the reference. Any output
+ * pointer may be NULL. */
+void tlm_schedule_tables(int waveform, float f0, float dt, int nsteps, float gd_width,
+                         float gd_delay, float *sv, float *cph, float *sph);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1233 @@
+//---------------------------------------------------------------------------
+// tlm_oracle.cpp - CPU restatement of the reference SCN TLM time step.
+//
+// TEST INFRASTRUCTURE ONLY. This file is the checker for the HIP path:
+// only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+// load it. The product library (emsolver_amd) never links or calls it.
+//
+// It restates TlmSolver's CPU path (TlmSolver.cpp in the reference) with
+// the same FP32 operation order, the same injection order, the same monitor
+// formulas and the same ThreadPool rule (min(16, hw-1) persistent workers,
+// z-slab chunks, separate scatter and connect passes). It is templated on
+// the arithmetic type: float is the parity oracle, double is the precision
+// oracle used for the reduced-precision pulse storage study.
+//
+// Each function cites the reference file:line it follows.
+//
+// Pinning status (how each function is checked, see tests/test_oracle_*.py):
+//   ScatterCell / ScatterCellDiel : unitarity bars of the reference
+//       self test, closed-form SCN properties, cavity physics  (pinned)
+//   connect                       : brute-force FP64 operator, energy
+//       invariants, PEC cavity resonance                      (pinned)
+//   inject / waveformValue        : closed forms                (pinned)
+//   monitors (probe, port, Js, Huygens, energy, plane)       : closed forms
+//       on hand-built states, post-hoc DFT identities          (pinned)
+//---------------------------------------------------------------------------
+#include "tlm_oracle.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace {
+
+// TlmSolver.cpp:15-16
+const float C0 = 299792458.0f;
+const float ETA0 = 376.730313f;
+
+// TlmSolver.cpp:20-29 port = axis*4 + side*2 + pol, POL[axis][pol] = E axis
+const int POL[3][2] = {{1, 2}, {0, 2}, {0, 1}};
+inline int portOf(int axis, int side, int polAxis)
+{
+    int pol = (polAxis == POL[axis][0]) ? 0 : 1;
+    return axis * 4 + side * 2 + pol;
+}
+// TlmSolver.cpp:33-34 cyclic pair (i,j) with e_i x e_j = e_k
+const int CYC_I[3] = {1, 2, 0};
+const int CYC_J[3] = {2, 0, 1};
+
+// TlmSolver.cpp:37-42 loop circulation about axis k
+template <class T> inline T ZIof(const T *vp, int k)
+{
+    int i = CYC_I[k], j = CYC_J[k];
+    return T(0.5) * (vp[portOf(i, 0, j)] - vp[portOf(i, 1, j)] -
+                     vp[portOf(j, 0, i)] + vp[portOf(j, 1, i)]);
+}
+
+// TlmSolver.cpp:46-52 node voltage for polarisation j
+template <class T> inline T VjOf(const T *vp, int j)
+{
+    int a1 = (j == 0) ? 1 : 0;
+    int a2 = (j == 2) ? 1 : 2;
+    return T(0.5) * (vp[portOf(a1, 0, j)] + vp[portOf(a1, 1, j)] +
+                     vp[portOf(a2, 0, j)] + vp[portOf(a2, 1, j)]);
+}
+
+// TlmSolver.cpp:59-76 air SCN scatter
+template <class T> inline void ScatterCell(const T *vin, T *vout)
+{
+    T Vp[3], ZI[3];
+    for (int j = 0; j < 3; ++j)
+        Vp[j] = VjOf(vin, j);
+    for (int k = 0; k < 3; ++k)
+        ZI[k] = ZIof(vin, k);
+    for (int i = 0; i < 3; ++i)
+        for (int s = 0; s < 2; ++s)
+            for (int pol = 0; pol < 2; ++pol)
+            {
+                int j = POL[i][pol];
+                int k = 3 - i - j;
+                T sg = (((i + 1) % 3) == j ? T(1) : T(-1)) * (s ? T(1) : T(-1));
+                vout[i * 4 + s * 2 + pol] = Vp[j] + sg * ZI[k] - vin[portOf(i, 1 - s, j)];
+            }
+}
+
+// TlmSolver.cpp:86-113 stub-loaded SCN with shunt loss
+template <class T> inline void ScatterCellDiel(const T *vin, T *vs, T yh, T gh, T *vout)
+{
+    T Vp[3], ZI[3];
+    T den = T(4) + yh + gh;
+    for (int j = 0; j < 3; ++j)
+    {
+        int a1 = (j == 0) ? 1 : 0;
+        int a2 = (j == 2) ? 1 : 2;
+        T sum4 = vin[portOf(a1, 0, j)] + vin[portOf(a1, 1, j)] +
+                 vin[portOf(a2, 0, j)] + vin[portOf(a2, 1, j)];
+        Vp[j] = T(2) * (sum4 + yh * vs[j]) / den;
+    }
+    for (int k = 0; k < 3; ++k)
+        ZI[k] = ZIof(vin, k);
+    for (int i = 0; i < 3; ++i)
+        for (int s = 0; s < 2; ++s)
+            for (int pol = 0; pol < 2; ++pol)
+            {
+                int j = POL[i][pol];
+                int k = 3 - i - j;
+                T sg = (((i + 1) % 3) == j ? T(1) : T(-1)) * (s ? T(1) : T(-1));
+                vout[i * 4 + s * 2 + pol] = Vp[j] + sg * ZI[k] - vin[portOf(i, 1 - s, j)];
+            }
+    for (int j = 0; j < 3; ++j)
+        vs[j] = Vp[j] - vs[j];
+}
+
+//---------------------------------------------------------------------------
+// Blocking pool: run(chunks, fn) runs fn(0..chunks-1) on persistent workers
+// (behaviour of ThreadPool, TlmSolver.h:34-54 / TlmSolver.cpp:174-240).
+//---------------------------------------------------------------------------
+class WorkPool
+{
+public:
+    explicit WorkPool(int n)
+    {
+        if (n < 1)
+            n = 1;
+        for (int i = 0; i < n; ++i)
+            threads_.emplace_back([this] { loop(); });
+    }
+    ~WorkPool()
+    {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            stop_ = true;
+            ++epoch_;
+        }
+        wake_.notify_all();
+        for (auto &t : threads_)
+            t.join();
+    }
+    int size() const { return (int)threads_.size(); }
+    void run(int chunks, const std::function<void(int)> &fn)
+    {
+        if (chunks <= 0)
+            return;
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            fn_ = &fn;
+            total_ = chunks;
+            next_ = 0;
+            finished_ = 0;
+            ++epoch_;
+        }
+        wake_.notify_all();
+        std::unique_lock<std::mutex> g(mu_);
+        done_.wait(g, [&] { return finished_ == (int)threads_.size(); });
+    }
+
+private:
+    void loop()
+    {
+        uint64_t seen = 0;
+        for (;;)
+        {
+            {
+                std::unique_lock<std::mutex> g(mu_);
+                wake_.wait(g, [&] { return stop_ || epoch_ != seen; });
+                if (stop_)
+                    return;
+                seen = epoch_;
+            }
+            for (int c; (c = next_.fetch_add(1)) < total_;)
+                (*fn_)(c);
+            std::lock_guard<std::mutex> g(mu_);
+            if (++finished_ == (int)threads_.size())
+                done_.notify_all();
+        }
+    }
+    std::vector<std::thread> threads_;
+    std::mutex mu_;
+    std::condition_variable wake_, done_;
+    const std::function<void(int)> *fn_ = nullptr;
+    std::atomic<int> next_{0};
+    int total_ = 0, finished_ = 0;
+    uint64_t epoch_ = 0;
+    bool stop_ = false;
+};
+
+struct Face
+{
+    int64_t airCell;
+    int axis, sign;
+    float cx, cy, cz;
+};
+struct HFace
+{
+    int64_t cell;
+    int axis, outSign;
+};
+struct Port
+{
+    std::vector<int64_t> cells;
+    int polAxis = 2;
+    float amp = 1.0f;
+};
+struct Source
+{
+    int64_t cell;
+    int polAxis;
+    float amp;
+};
+
+//---------------------------------------------------------------------------
+// waveformValue, TlmSolver.cpp:410-436 (float libm), plus the
+// Gaussian-derivative pulse used by the benchmark scenes (defined on the
+// step index: sv = -sqrt(2e) u exp(-u^2), u = (n - delay)/width, evaluated
+// in double and rounded to float; peak magnitude 1).
+//---------------------------------------------------------------------------
+float waveform(int type, float f0, float dt, int n, float gdWidth, float gdDelay)
+{
+    if (type == OT_WF_GAUSS_DERIV)
+    {
+        double u = ((double)n - (double)gdDelay) / (double)gdWidth;
+        return (float)(-std::sqrt(2.0 * std::exp(1.0)) * u * std::exp(-u * u));
+    }
+    float t = n * dt;
+    float w = 2.0f * (float)M_PI * f0;
+    switch (type)
+    {
+    case OT_WF_CW_RAMPED:
+    {
+        float Tr = 3.0f / f0;
+        float r = (t < Tr) ? 0.5f * (1.0f - std::cos((float)M_PI * t / Tr)) : 1.0f;
+        return r * std::sin(w * t);
+    }
+    case OT_WF_GAUSS_SINE:
+    {
+        float tau = 2.0f / f0, t0 = 3.0f * tau;
+        float u = (t - t0) / tau;
+        return std::exp(-u * u) * std::sin(w * (t - t0));
+    }
+    case OT_WF_GAUSS_PULSE:
+    default:
+    {
+        float tau = 0.5f / f0, t0 = 3.0f * tau;
+        float u = (t - t0) / tau;
+        return std::exp(-u * u);
+    }
+    }
+}
+
+//---------------------------------------------------------------------------
+struct SolverBase
+{
+    virtual ~SolverBase() {}
+};
+
+template <class T> struct Solver : SolverBase
+{
+    // grid (VoxelGridSpec, Geometry.h:85-113)
+    int nx = 0, ny = 0, nz = 0;
+    float ox = 0, oy = 0, oz = 0, dl = 1e-3f;
+    // config (TlmConfig, TlmSolver.h:113-123)
+    float f0 = 1e9f;
+    int wfType = OT_WF_CW_RAMPED;
+    int totalSteps = 0, settleSteps = 0, huyOffset = 0, recordEvery = 0;
+    float rho = 0.0f;
+    float gdWidth = 20.0f, gdDelay = 60.0f;
+
+    std::vector<uint8_t> mat;
+    std::vector<T> yhat, ghat;
+    std::vector<T> V, Vs;
+    std::vector<Source> sources;
+    std::vector<Port> ports;
+    std::vector<std::vector<double>> portV, portI;     // values of type T
+    std::vector<int64_t> probeCells;
+    std::vector<std::vector<T>> probeE;         // per probe: [step][3]
+    std::vector<double> dftFreqs;
+    std::vector<std::vector<double>> probeDftRe, probeDftIm;   // per probe: [3][nf]
+    std::vector<Face> faces;
+    std::vector<HFace> hfaces;
+    std::vector<T> jsInstant, jsDftRe, jsDftIm, jsDftMag, huyDft;
+    int dftSamples = 0, huySamples = 0;
+    int planeAxis = -1, planeIndex = 0, planeN1 = 0, planeN2 = 0;
+    std::vector<T> planeBuf;
+    std::vector<int> frameSteps;
+    std::vector<std::vector<T>> frameJs, framePlane;
+    std::vector<int> energySteps;
+    std::vector<float> energyVals;
+    std::vector<double> energyD;
+    int curStep = 0;
+    float dt = 0.0f;
+    std::unique_ptr<WorkPool> pool;
+
+    size_t ncell() const { return (size_t)nx * ny * nz; }
+    size_t cellIndex(int i, int j, int k) const { return ((size_t)k * ny + j) * nx + i; }
+
+    // TlmSolver.cpp:259-304
+    void setup(const uint8_t *m, const float *epsr, const float *sigma, int nmat)
+    {
+        size_t n = ncell();
+        mat.assign(m, m + n);
+        V.assign(n * 12, T(0));
+        dt = dl / (2.0f * C0);
+        yhat.clear();
+        ghat.clear();
+        for (int i = 0; i < nmat; ++i)
+        {
+            yhat.push_back(T(4) * (std::max(T(1), T(epsr[i])) - T(1)));
+            ghat.push_back(std::max(T(0), T(sigma[i])) * T(dl) * T(ETA0));
+        }
+        bool anyDiel = false;
+        for (uint8_t mm : mat)
+            if (mm >= 2)
+            {
+                anyDiel = true;
+                break;
+            }
+        if (anyDiel)
+            Vs.assign(n * 3, T(0));
+        else
+            Vs.clear();
+        curStep = 0;
+        dftSamples = huySamples = 0;
+        buildSurfaceFaces();
+        buildHuygens();
+        jsInstant.assign(faces.size(), T(0));
+        jsDftRe.assign(faces.size() * 2, T(0));
+        jsDftIm.assign(faces.size() * 2, T(0));
+        jsDftMag.clear();
+    }
+
+    // BuildPecSurfaceFaces, Geometry.cpp:402-436 (k, j, i, axis, side order)
+    void buildSurfaceFaces()
+    {
+        faces.clear();
+        const long long dirs[3] = {1, nx, (long long)nx * ny};
+        const int nn[3] = {nx, ny, nz};
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i)
+                {
+                    size_t c = cellIndex(i, j, k);
+                    if (mat[c] != 1)
+                        continue;
+                    const int co[3] = {i, j, k};
+                    for (int a = 0; a < 3; ++a)
+                        for (int s = -1; s <= 1; s += 2)
+                        {
+                            int q = co[a] + s;
+                            if (q < 0 || q >= nn[a])
+                                continue;
+                            size_t nb = c + (size_t)(s * dirs[a]);
+                            if (mat[nb] == 1)
+                                continue;
+                            Face f;
+                            f.airCell = (int64_t)nb;
+                            f.axis = a;
+                            f.sign = s;
+                            float cc[3] = {ox + (i + 0.5f) * dl, oy + (j + 0.5f) * dl,
+                                           oz + (k + 0.5f) * dl};
+                            cc[a] = cc[a] + s * dl * 0.5f;
+                            f.cx = cc[0];
+                            f.cy = cc[1];
+                            f.cz = cc[2];
+                            faces.push_back(f);
+                        }
+                }
+    }
+
+    // TlmSolver.cpp:313-359
+    void buildHuygens()
+    {
+        hfaces.clear();
+        huyDft.clear();
+        int o = huyOffset;
+        if (o < 2)
+            return;
+        const int nn[3] = {nx, ny, nz};
+        if (2 * o + 2 >= nn[0] || 2 * o + 2 >= nn[1] || 2 * o + 2 >= nn[2])
+            return;
+        auto add = [&](int i, int j, int k, int axis, int outSign) {
+            size_t c = cellIndex(i, j, k);
+            if (mat[c] != 0)
+                return;
+            hfaces.push_back({(int64_t)c, axis, outSign});
+        };
+        int x0 = o, x1 = nn[0] - 1 - o, y0 = o, y1 = nn[1] - 1 - o, z0 = o, z1 = nn[2] - 1 - o;
+        for (int k = z0; k <= z1; ++k)
+            for (int j = y0; j <= y1; ++j)
+            {
+                add(x0, j, k, 0, -1);
+                add(x1, j, k, 0, +1);
+            }
+        for (int k = z0; k <= z1; ++k)
+            for (int i = x0; i <= x1; ++i)
+            {
+                add(i, y0, k, 1, -1);
+                add(i, y1, k, 1, +1);
+            }
+        for (int j = y0; j <= y1; ++j)
+            for (int i = x0; i <= x1; ++i)
+            {
+                add(i, j, z0, 2, -1);
+                add(i, j, z1, 2, +1);
+            }
+        huyDft.assign(hfaces.size() * 8, T(0));
+    }
+
+    // TlmSolver.cpp:362-374 (keeps AIR cells only)
+    int addPort(const int64_t *cells, int n, int polAxis, float amp)
+    {
+        Port p;
+        p.polAxis = polAxis;
+        p.amp = amp;
+        for (int i = 0; i < n; ++i)
+            if (cells[i] >= 0 && (size_t)cells[i] < mat.size() && mat[cells[i]] == 0)
+                p.cells.push_back(cells[i]);
+        ports.push_back(p);
+        portV.emplace_back();
+        portI.emplace_back();
+        return (int)ports.size() - 1;
+    }
+
+    // TlmSolver.cpp:377-391 (cellOf uses floor, no clamping)
+    int addProbe(float x, float y, float z)
+    {
+        int i = (int)std::floor((x - ox) / dl);
+        int j = (int)std::floor((y - oy) / dl);
+        int k = (int)std::floor((z - oz) / dl);
+        if (!(i >= 0 && j >= 0 && k >= 0 && i < nx && j < ny && k < nz))
+            return -1;
+        return addProbeCell((int64_t)cellIndex(i, j, k));
+    }
+    int addProbeCell(int64_t cell)
+    {
+        if (cell < 0 || (size_t)cell >= ncell())
+            return -1;
+        probeCells.push_back(cell);
+        probeE.emplace_back();
+        probeDftRe.emplace_back(3 * dftFreqs.size(), 0.0);
+        probeDftIm.emplace_back(3 * dftFreqs.size(), 0.0);
+        return (int)probeCells.size() - 1;
+    }
+    void setProbeDft(const double *f, int nf)
+    {
+        dftFreqs.assign(f, f + nf);
+        for (auto &r : probeDftRe)
+            r.assign(3 * nf, 0.0);
+        for (auto &r : probeDftIm)
+            r.assign(3 * nf, 0.0);
+    }
+
+    // TlmSolver.cpp:394-407 (AIR cells of the plane only)
+    void addPlaneWave(int propAxis, int planeIdx, int polAxis, float amp)
+    {
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i)
+                {
+                    int co[3] = {i, j, k};
+                    if (co[propAxis] != planeIdx)
+                        continue;
+                    size_t c = cellIndex(i, j, k);
+                    if (mat[c] == 0)
+                        sources.push_back({(int64_t)c, polAxis, amp});
+                }
+    }
+    void addSource(int64_t cell, int polAxis, float amp) { sources.push_back({cell, polAxis, amp}); }
+
+    float waveformValue(int n) const { return waveform(wfType, f0, dt, n, gdWidth, gdDelay); }
+
+    // TlmSolver.cpp:457-464
+    void runCpu()
+    {
+        for (int n = curStep; n < totalSteps; ++n)
+        {
+            step(n);
+            curStep = n + 1;
+        }
+    }
+
+    // TlmSolver.cpp:467-482
+    void finalizeDft()
+    {
+        jsDftMag.assign(faces.size(), T(0));
+        if (dftSamples > 0)
+        {
+            T norm = T(2) / T(dftSamples);
+            for (size_t f = 0; f < faces.size(); ++f)
+            {
+                T r1 = jsDftRe[f * 2], i1 = jsDftIm[f * 2];
+                T r2 = jsDftRe[f * 2 + 1], i2 = jsDftIm[f * 2 + 1];
+                jsDftMag[f] = norm * std::sqrt(r1 * r1 + i1 * i1 + r2 * r2 + i2 * i2);
+            }
+        }
+    }
+
+    // TlmSolver.cpp:485-513: inject (sources, then port cells) -> monitors
+    // -> scatter -> connect
+    void step(int n)
+    {
+        float svf = waveformValue(n);
+        T sv = T(svf);
+        auto inject = [&](int64_t cell, int polAxis, float amp) {
+            T q = T(0.5) * T(amp) * sv;
+            T *vp = &V[(size_t)cell * 12];
+            for (int axis = 0; axis < 3; ++axis)
+            {
+                if (axis == polAxis)
+                    continue;
+                vp[portOf(axis, 0, polAxis)] += q;
+                vp[portOf(axis, 1, polAxis)] += q;
+            }
+        };
+        for (const auto &s : sources)
+            inject(s.cell, s.polAxis, s.amp);
+        for (const auto &p : ports)
+            for (int64_t c : p.cells)
+                inject(c, p.polAxis, p.amp);
+        monitors(n);
+        scatter();
+        connect();
+    }
+
+    // TlmSolver.cpp:516-548
+    void scatter()
+    {
+        int chunks = pool->size();
+        bool haveStubs = !Vs.empty();
+        pool->run(chunks, [&](int c) {
+            int k0 = (int)((long long)nz * c / chunks);
+            int k1 = (int)((long long)nz * (c + 1) / chunks);
+            size_t slab = (size_t)nx * ny;
+            for (int k = k0; k < k1; ++k)
+            {
+                size_t cell = slab * k;
+                for (size_t q = 0; q < slab; ++q, ++cell)
+                {
+                    uint8_t mm = mat[cell];
+                    if (mm == 1)
+                        continue;
+                    T vin[12], vout[12];
+                    std::memcpy(vin, &V[cell * 12], sizeof(vin));
+                    if (mm >= 2 && haveStubs)
+                    {
+                        int id = mm - 2;
+                        ScatterCellDiel(vin, &Vs[cell * 3], yhat[id], ghat[id], vout);
+                    }
+                    else
+                        ScatterCell(vin, vout);
+                    std::memcpy(&V[cell * 12], vout, sizeof(vout));
+                }
+            }
+        });
+    }
+
+    // TlmSolver.cpp:551-613 (in-place swaps, PEC short circuit, outer rho)
+    void connect()
+    {
+        const size_t sx = 1, sy = (size_t)nx, sz = (size_t)nx * ny;
+        const T r = T(rho);
+        int chunks = pool->size();
+        pool->run(chunks, [&](int ch) {
+            int k0 = (int)((long long)nz * ch / chunks);
+            int k1 = (int)((long long)nz * (ch + 1) / chunks);
+            auto link = [&](size_t c, size_t nb, int axis) {
+                bool ca = (mat[c] != 1), na = (mat[nb] != 1);
+                T *vc = &V[c * 12 + (size_t)axis * 4];
+                T *vn = &V[nb * 12 + (size_t)axis * 4];
+                for (int pol = 0; pol < 2; ++pol)
+                {
+                    if (ca && na)
+                        std::swap(vc[2 + pol], vn[pol]);
+                    else if (ca)
+                        vc[2 + pol] = -vc[2 + pol];
+                    else if (na)
+                        vn[pol] = -vn[pol];
+                }
+            };
+            auto outer = [&](size_t c, int axis, int side) {
+                T *vp = &V[c * 12 + (size_t)axis * 4 + (size_t)side * 2];
+                vp[0] *= r;
+                vp[1] *= r;
+            };
+            for (int k = k0; k < k1; ++k)
+                for (int j = 0; j < ny; ++j)
+                {
+                    size_t row = sz * k + sy * j;
+                    for (int i = 0; i < nx - 1; ++i)
+                        link(row + i, row + i + sx, 0);
+                    outer(row, 0, 0);
+                    outer(row + (nx - 1), 0, 1);
+                    if (j < ny - 1)
+                        for (int i = 0; i < nx; ++i)
+                            link(row + i, row + i + sy, 1);
+                    else
+                        for (int i = 0; i < nx; ++i)
+                            outer(row + i, 1, 1);
+                    if (j == 0)
+                        for (int i = 0; i < nx; ++i)
+                            outer(row + i, 1, 0);
+                    if (k < nz - 1)
+                        for (int i = 0; i < nx; ++i)
+                            link(row + i, row + i + sz, 2);
+                    else
+                        for (int i = 0; i < nx; ++i)
+                            outer(row + i, 2, 1);
+                    if (k == 0)
+                        for (int i = 0; i < nx; ++i)
+                            outer(row + i, 2, 0);
+                }
+        });
+    }
+
+    // TlmSolver.cpp:616-790, plus the FP64 probe DFT (new monitor)
+    void monitors(int n)
+    {
+        const bool doDft = (n >= settleSteps);
+        const float phase = 2.0f * (float)M_PI * f0 * (n * dt);
+        const T cph = T(std::cos(phase)), sph = T(std::sin(phase));
+        const int nf = (int)faces.size();
+        int chunks = pool->size();
+
+        // surface currents, TlmSolver.cpp:627-649
+        pool->run(chunks, [&](int c) {
+            int a = (int)((long long)nf * c / chunks);
+            int b = (int)((long long)nf * (c + 1) / chunks);
+            for (int f = a; f < b; ++f)
+            {
+                const Face &sf = faces[f];
+                const T *vp = &V[(size_t)sf.airCell * 12];
+                int t1 = POL[sf.axis][0], t2 = POL[sf.axis][1];
+                T h1 = ZIof(vp, t1), h2 = ZIof(vp, t2);
+                jsInstant[f] = std::sqrt(h1 * h1 + h2 * h2);
+                if (doDft)
+                {
+                    jsDftRe[f * 2] += h1 * cph;
+                    jsDftIm[f * 2] -= h1 * sph;
+                    jsDftRe[f * 2 + 1] += h2 * cph;
+                    jsDftIm[f * 2 + 1] -= h2 * sph;
+                }
+            }
+        });
+        if (doDft)
+            ++dftSamples;
+
+        // Huygens DFT, TlmSolver.cpp:651-675
+        if (doDft && !hfaces.empty())
+        {
+            const int nh = (int)hfaces.size();
+            pool->run(chunks, [&](int c) {
+                int a = (int)((long long)nh * c / chunks);
+                int b = (int)((long long)nh * (c + 1) / chunks);
+                for (int f = a; f < b; ++f)
+                {
+                    const HFace &hf = hfaces[f];
+                    const T *vp = &V[(size_t)hf.cell * 12];
+                    int t1 = POL[hf.axis][0], t2 = POL[hf.axis][1];
+                    T e1 = -VjOf(vp, t1), e2 = -VjOf(vp, t2);
+                    T m1 = -ZIof(vp, t1), m2 = -ZIof(vp, t2);
+                    T *acc = &huyDft[(size_t)f * 8];
+                    acc[0] += e1 * cph;
+                    acc[1] -= e1 * sph;
+                    acc[2] += e2 * cph;
+                    acc[3] -= e2 * sph;
+                    acc[4] += m1 * cph;
+                    acc[5] -= m1 * sph;
+                    acc[6] += m2 * cph;
+                    acc[7] -= m2 * sph;
+                }
+            });
+            ++huySamples;
+        }
+
+        // port V/I, TlmSolver.cpp:677-703
+        const size_t strideOf[3] = {1, (size_t)nx, (size_t)nx * ny};
+        for (size_t pi = 0; pi < ports.size(); ++pi)
+        {
+            const Port &p = ports[pi];
+            T vSum = T(0);
+            for (int64_t c : p.cells)
+                vSum += VjOf(&V[(size_t)c * 12], p.polAxis);
+            T iVal = T(0);
+            if (!p.cells.empty())
+            {
+                size_t c = (size_t)p.cells[p.cells.size() / 2];
+                int a = p.polAxis;
+                int t1 = CYC_I[a], t2 = CYC_J[a];
+                size_t s1 = strideOf[t1], s2 = strideOf[t2];
+                T zi = T(0);
+                zi += ZIof(&V[(c + s1) * 12], t2);
+                zi -= ZIof(&V[(c - s1) * 12], t2);
+                zi -= ZIof(&V[(c + s2) * 12], t1);
+                zi += ZIof(&V[(c - s2) * 12], t1);
+                iVal = T(-2) * zi / T(ETA0);
+            }
+            portV[pi].push_back((double)vSum);
+            portI[pi].push_back((double)iVal);
+        }
+
+        // E probes, TlmSolver.cpp:705-713, then the running FP64 DFT
+        for (size_t pi = 0; pi < probeCells.size(); ++pi)
+        {
+            const T *vp = &V[(size_t)probeCells[pi] * 12];
+            T inv = T(-1) / T(dl);
+            T e[3] = {inv * VjOf(vp, 0), inv * VjOf(vp, 1), inv * VjOf(vp, 2)};
+            for (int j = 0; j < 3; ++j)
+                probeE[pi].push_back(e[j]);
+            const int nfq = (int)dftFreqs.size();
+            for (int q = 0; q < nfq; ++q)
+            {
+                double ph = 2.0 * M_PI * dftFreqs[q] * ((double)n * (double)dt);
+                double cs = std::cos(ph), sn = std::sin(ph);
+                for (int j = 0; j < 3; ++j)
+                {
+                    probeDftRe[pi][j * nfq + q] += (double)e[j] * cs;
+                    probeDftIm[pi][j * nfq + q] -= (double)e[j] * sn;
+                }
+            }
+        }
+
+        // |E| cut plane every second step, TlmSolver.cpp:715-745
+        int pa = planeAxis;
+        if (pa >= 0 && (n % 2) == 0)
+        {
+            const int nn[3] = {nx, ny, nz};
+            int a1 = (pa == 0) ? 1 : 0;
+            int a2 = (pa == 2) ? 1 : 2;
+            planeN1 = nn[a1];
+            planeN2 = nn[a2];
+            planeBuf.assign((size_t)planeN1 * planeN2, T(0));
+            int idx = std::min(std::max(planeIndex, 0), nn[pa] - 1);
+            for (int q2 = 0; q2 < planeN2; ++q2)
+                for (int q1 = 0; q1 < planeN1; ++q1)
+                {
+                    int co[3];
+                    co[pa] = idx;
+                    co[a1] = q1;
+                    co[a2] = q2;
+                    size_t cell = cellIndex(co[0], co[1], co[2]);
+                    if (mat[cell] == 1)
+                        continue;
+                    const T *vp = &V[cell * 12];
+                    T e2acc = T(0);
+                    for (int j = 0; j < 3; ++j)
+                    {
+                        T vj = VjOf(vp, j);
+                        e2acc += vj * vj;
+                    }
+                    planeBuf[(size_t)q2 * planeN1 + q1] = std::sqrt(e2acc);
+                }
+        }
+
+        // playback frames, TlmSolver.cpp:747-763
+        if (recordEvery > 0 && (n % recordEvery) == 0)
+        {
+            frameSteps.push_back(n);
+            frameJs.push_back(jsInstant);
+            framePlane.push_back((pa >= 0) ? planeBuf : std::vector<T>());
+        }
+
+        // link-line energy every 16 steps, TlmSolver.cpp:766-789
+        if ((n & 15) == 0)
+        {
+            std::vector<double> part(chunks, 0.0);
+            const size_t total = V.size();
+            pool->run(chunks, [&](int c) {
+                size_t a = total * c / chunks, b = total * (c + 1) / chunks;
+                double s = 0.0;
+                for (size_t i = a; i < b; ++i)
+                    s += (double)V[i] * V[i];
+                part[c] = s;
+            });
+            double e = 0.0;
+            for (double p : part)
+                e += p;
+            energySteps.push_back(n);
+            energyVals.push_back((float)e);
+            energyD.push_back(e);
+        }
+    }
+};
+
+template <class T> Solver<T> *as(void *h) { return static_cast<Solver<T> *>(h); }
+
+struct Handle
+{
+    int fp64;
+    SolverBase *s;
+};
+
+template <class F> auto dispatch(ot_handle *h, F &&f)
+{
+    Handle *hh = reinterpret_cast<Handle *>(h);
+    if (hh->fp64)
+        return f(*static_cast<Solver<double> *>(hh->s));
+    return f(*static_cast<Solver<float> *>(hh->s));
+}
+
+template <class T> void copy_out(const std::vector<T> &src, double *dst)
+{
+    for (size_t i = 0; i < src.size(); ++i)
+        dst[i] = (double)src[i];
+}
+
+} // namespace
+
+//---------------------------------------------------------------------------
+// C API (ctypes)
+//---------------------------------------------------------------------------
+extern "C" {
+
+int ot_default_threads(void)
+{
+    // TlmSolver.cpp:247-249
+    int n = (int)std::thread::hardware_concurrency();
+    return std::max(1, std::min(16, n - 1));
+}
+
+float ot_waveform_value(int type, float f0, float dt, int n, float gdWidth, float gdDelay)
+{
+    return waveform(type, f0, dt, n, gdWidth, gdDelay);
+}
+
+ot_handle *ot_create(const ot_config *cfg, const uint8_t *mat, const float *epsr,
+                     const float *sigma, int nmat)
+{
+    Handle *h = new Handle;
+    h->fp64 = cfg->fp64;
+    auto init = [&](auto *s) {
+        s->nx = cfg->nx;
+        s->ny = cfg->ny;
+        s->nz = cfg->nz;
+        s->dl = cfg->dl;
+        s->ox = cfg->ox;
+        s->oy = cfg->oy;
+        s->oz = cfg->oz;
+        s->f0 = cfg->f0;
+        s->wfType = cfg->waveform;
+        s->totalSteps = cfg->total_steps;
+        s->settleSteps = cfg->settle_steps;
+        s->rho = cfg->rho;
+        s->huyOffset = cfg->huy_offset;
+        s->recordEvery = cfg->record_every;
+        s->gdWidth = cfg->gd_width;
+        s->gdDelay = cfg->gd_delay;
+        int nt = cfg->threads > 0 ? cfg->threads : ot_default_threads();
+        s->pool.reset(new WorkPool(nt));
+        s->setup(mat, epsr, sigma, nmat);
+        h->s = s;
+    };
+    if (cfg->fp64)
+        init(new Solver<double>());
+    else
+        init(new Solver<float>());
+    return reinterpret_cast<ot_handle *>(h);
+}
+
+void ot_destroy(ot_handle *h)
+{
+    if (!h)
+        return;
+    Handle *hh = reinterpret_cast<Handle *>(h);
+    delete hh->s;
+    delete hh;
+}
+
+int ot_add_port(ot_handle *h, const int64_t *cells, int n, int pol, float amp)
+{
+    return dispatch(h, [&](auto &s) { return s.addPort(cells, n, pol, amp); });
+}
+void ot_add_plane_wave(ot_handle *h, int propAxis, int planeIndex, int polAxis, float amp)
+{
+    dispatch(h, [&](auto &s) {
+        s.addPlaneWave(propAxis, planeIndex, polAxis, amp);
+        return 0;
+    });
+}
+void ot_add_source(ot_handle *h, int64_t cell, int pol, float amp)
+{
+    dispatch(h, [&](auto &s) {
+        s.addSource(cell, pol, amp);
+        return 0;
+    });
+}
+int ot_add_probe(ot_handle *h, float x, float y, float z)
+{
+    return dispatch(h, [&](auto &s) { return s.addProbe(x, y, z); });
+}
+int ot_add_probe_cell(ot_handle *h, int64_t cell)
+{
+    return dispatch(h, [&](auto &s) { return s.addProbeCell(cell); });
+}
+void ot_set_probe_dft(ot_handle *h, const double *freqs, int nf)
+{
+    dispatch(h, [&](auto &s) {
+        s.setProbeDft(freqs, nf);
+        return 0;
+    });
+}
+void ot_set_field_plane(ot_handle *h, int axis, int index)
+{
+    dispatch(h, [&](auto &s) {
+        s.planeAxis = axis;
+        s.planeIndex = index;
+        return 0;
+    });
+}
+void ot_set_state(ot_handle *h, const double *V, const double *Vs)
+{
+    dispatch(h, [&](auto &s) {
+        using T = typename std::remove_reference<decltype(s.V[0])>::type;
+        for (size_t i = 0; i < s.V.size(); ++i)
+            s.V[i] = (T)V[i];
+        if (Vs)
+            for (size_t i = 0; i < s.Vs.size(); ++i)
+                s.Vs[i] = (T)Vs[i];
+        return 0;
+    });
+}
+void ot_run(ot_handle *h)
+{
+    dispatch(h, [&](auto &s) {
+        s.runCpu();
+        s.finalizeDft();
+        return 0;
+    });
+}
+void ot_run_steps(ot_handle *h, int count)
+{
+    dispatch(h, [&](auto &s) {
+        int end = std::min(s.totalSteps, s.curStep + count);
+        for (int n = s.curStep; n < end; ++n)
+        {
+            s.step(n);
+            s.curStep = n + 1;
+        }
+        return 0;
+    });
+}
+void ot_scatter_connect(ot_handle *h)
+{
+    dispatch(h, [&](auto &s) {
+        s.scatter();
+        s.connect();
+        return 0;
+    });
+}
+void ot_finalize(ot_handle *h)
+{
+    dispatch(h, [&](auto &s) {
+        s.finalizeDft();
+        return 0;
+    });
+}
+int ot_cur_step(ot_handle *h)
+{
+    return dispatch(h, [&](auto &s) { return s.curStep; });
+}
+int ot_has_stubs(ot_handle *h)
+{
+    return dispatch(h, [&](auto &s) { return (int)!s.Vs.empty(); });
+}
+float ot_dt(ot_handle *h)
+{
+    return dispatch(h, [&](auto &s) { return s.dt; });
+}
+void ot_get_state(ot_handle *h, double *V, double *Vs)
+{
+    dispatch(h, [&](auto &s) {
+        copy_out(s.V, V);
+        if (Vs)
+            copy_out(s.Vs, Vs);
+        return 0;
+    });
+}
+int ot_num_faces(ot_handle *h)
+{
+    return dispatch(h, [&](auto &s) { return (int)s.faces.size(); });
+}
+void ot_get_faces(ot_handle *h, int64_t *airCell, int32_t *axis, int32_t *sign, float *center)
+{
+    dispatch(h, [&](auto &s) {
+        for (size_t f = 0; f < s.faces.size(); ++f)
+        {
+            airCell[f] = s.faces[f].airCell;
+            axis[f] = s.faces[f].axis;
+            sign[f] = s.faces[f].sign;
+            if (center)
+            {
+                center[3 * f] = s.faces[f].cx;
+                center[3 * f + 1] = s.faces[f].cy;
+                center[3 * f + 2] = s.faces[f].cz;
+            }
+        }
+        return 0;
+    });
+}
+int ot_num_huygens(ot_handle *h)
+{
+    return dispatch(h, [&](auto &s) { return (int)s.hfaces.size(); });
+}
+void ot_get_huygens_faces(ot_handle *h, int64_t *cell, int32_t *axis, int32_t *outSign)
+{
+    dispatch(h, [&](auto &s) {
+        for (size_t f = 0; f < s.hfaces.size(); ++f)
+        {
+            cell[f] = s.hfaces[f].cell;
+            axis[f] = s.hfaces[f].axis;
+            outSign[f] = s.hfaces[f].outSign;
+        }
+        return 0;
+    });
+}
+void ot_get_js(ot_handle *h, double *inst, double *dftRe, double *dftIm, double *mag, int *samples)
+{
+    dispatch(h, [&](auto &s) {
+        copy_out(s.jsInstant, inst);
+        copy_out(s.jsDftRe, dftRe);
+        copy_out(s.jsDftIm, dftIm);
+        if (mag)
+            copy_out(s.jsDftMag, mag);
+        *samples = s.dftSamples;
+        return 0;
+    });
+}
+void ot_get_huygens(ot_handle *h, double *dft, int *samples)
+{
+    dispatch(h, [&](auto &s) {
+        copy_out(s.huyDft, dft);
+        *samples = s.huySamples;
+        return 0;
+    });
+}
+int ot_num_ports(ot_handle *h)
+{
+    return dispatch(h, [&](auto &s) { return (int)s.ports.size(); });
+}
+int ot_port_cells(ot_handle *h, int p, int64_t *cells)
+{
+    return dispatch(h, [&](auto &s) {
+        const auto &c = s.ports[p].cells;
+        if (cells)
+            std::copy(c.begin(), c.end(), cells);
+        return (int)c.size();
+    });
+}
+int ot_port_records(ot_handle *h, int p, double *v, double *i)
+{
+    return dispatch(h, [&](auto &s) {
+        const auto &a = s.portV[p], &b = s.portI[p];
+        if (v)
+            std::copy(a.begin(), a.end(), v);
+        if (i)
+            std::copy(b.begin(), b.end(), i);
+        return (int)a.size();
+    });
+}
+int ot_num_sources(ot_handle *h)
+{
+    return dispatch(h, [&](auto &s) { return (int)s.sources.size(); });
+}
+void ot_get_sources(ot_handle *h, int64_t *cell, int32_t *pol, float *amp)
+{
+    dispatch(h, [&](auto &s) {
+        for (size_t i = 0; i < s.sources.size(); ++i)
+        {
+            cell[i] = s.sources[i].cell;
+            pol[i] = s.sources[i].polAxis;
+            amp[i] = s.sources[i].amp;
+        }
+        return 0;
+    });
+}
+int ot_probe_cell(ot_handle *h, int p)
+{
+    return dispatch(h, [&](auto &s) { return (int)s.probeCells[p]; });
+}
+int ot_probe_series(ot_handle *h, int p, double *e)
+{
+    return dispatch(h, [&](auto &s) {
+        const auto &v = s.probeE[p];
+        if (e)
+            copy_out(v, e);
+        return (int)(v.size() / 3);
+    });
+}
+void ot_probe_dft(ot_handle *h, int p, double *re, double *im)
+{
+    dispatch(h, [&](auto &s) {
+        std::copy(s.probeDftRe[p].begin(), s.probeDftRe[p].end(), re);
+        std::copy(s.probeDftIm[p].begin(), s.probeDftIm[p].end(), im);
+        return 0;
+    });
+}
+int ot_energy(ot_handle *h, int32_t *steps, float *vals, double *dvals)
+{
+    return dispatch(h, [&](auto &s) {
+        for (size_t i = 0; i < s.energySteps.size(); ++i)
+        {
+            if (steps)
+                steps[i] = s.energySteps[i];
+            if (vals)
+                vals[i] = s.energyVals[i];
+            if (dvals)
+                dvals[i] = s.energyD[i];
+        }
+        return (int)s.energySteps.size();
+    });
+}
+int ot_plane(ot_handle *h, double *buf, int *n1, int *n2)
+{
+    return dispatch(h, [&](auto &s) {
+        *n1 = s.planeN1;
+        *n2 = s.planeN2;
+        if (buf)
+            copy_out(s.planeBuf, buf);
+        return (int)s.planeBuf.size();
+    });
+}
+int ot_num_frames(ot_handle *h)
+{
+    return dispatch(h, [&](auto &s) { return (int)s.frameSteps.size(); });
+}
+int ot_frame(ot_handle *h, int idx, double *js, double *plane)
+{
+    return dispatch(h, [&](auto &s) {
+        if (js)
+            copy_out(s.frameJs[idx], js);
+        if (plane)
+            copy_out(s.framePlane[idx], plane);
+        return s.frameSteps[idx];
+    });
+}
+
+// Scatter of a single node (TlmSolver.cpp:59-113), for the closed-form
+// checks: stub < 0 selects the air node.
+void ot_scatter_cell(const double *vin, double *vs, double yh, double gh, int diel, int fp64,
+                     double *vout)
+{
+    if (fp64)
+    {
+        if (diel)
+            ScatterCellDiel<double>(vin, vs, yh, gh, vout);
+        else
+            ScatterCell<double>(vin, vout);
+        return;
+    }
+    float fi[12], fo[12], fs[3];
+    for (int i = 0; i < 12; ++i)
+        fi[i] = (float)vin[i];
+    if (diel)
+    {
+        for (int i = 0; i < 3; ++i)
+            fs[i] = (float)vs[i];
+        ScatterCellDiel<float>(fi, fs, (float)yh, (float)gh, fo);
+        for (int i = 0; i < 3; ++i)
+            vs[i] = fs[i];
+    }
+    else
+        ScatterCell<float>(fi, fo);
+    for (int i = 0; i < 12; ++i)
+        vout[i] = fo[i];
+}
+
+// ScatterUnitarityError / DielScatterUnitarityError, TlmSolver.cpp:116-169
+float ot_scatter_unitarity_error(void)
+{
+    float S[12][12];
+    for (int c = 0; c < 12; ++c)
+    {
+        float vin[12] = {0}, vout[12];
+        vin[c] = 1.0f;
+        ScatterCell<float>(vin, vout);
+        for (int r = 0; r < 12; ++r)
+            S[r][c] = vout[r];
+    }
+    float maxErr = 0.0f;
+    for (int a = 0; a < 12; ++a)
+        for (int b = 0; b < 12; ++b)
+        {
+            float dot = 0.0f;
+            for (int r = 0; r < 12; ++r)
+                dot += S[r][a] * S[r][b];
+            float want = (a == b) ? 1.0f : 0.0f;
+            maxErr = std::max(maxErr, std::fabs(dot - want));
+        }
+    return maxErr;
+}
+
+float ot_diel_scatter_unitarity_error(float epsr)
+{
+    const int NP = 15;
+    float yh = 4.0f * (epsr - 1.0f);
+    float W[NP];
+    for (int i = 0; i < 12; ++i)
+        W[i] = 1.0f;
+    for (int i = 12; i < 15; ++i)
+        W[i] = yh;
+    float S[NP][NP];
+    for (int c = 0; c < NP; ++c)
+    {
+        float vin[12] = {0}, vs[3] = {0}, vout[12];
+        if (c < 12)
+            vin[c] = 1.0f;
+        else
+            vs[c - 12] = 1.0f;
+        ScatterCellDiel<float>(vin, vs, yh, 0.0f, vout);
+        for (int r = 0; r < 12; ++r)
+            S[r][c] = vout[r];
+        for (int r = 0; r < 3; ++r)
+            S[12 + r][c] = vs[r];
+    }
+    float maxErr = 0.0f;
+    for (int a = 0; a < NP; ++a)
+        for (int b = 0; b < NP; ++b)
+        {
+            float dot = 0.0f;
+            for (int r = 0; r < NP; ++r)
+                dot += W[r] * S[r][a] * S[r][b];
+            float want = (a == b) ? W[a] : 0.0f;
+            maxErr = std::max(maxErr, std::fabs(dot - want));
+        }
+    return maxErr;
+}
+
+} // extern "C"

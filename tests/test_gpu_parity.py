@@ -1,0 +1,114 @@
+"""GPU parity: the HIP step (through the C ABI) against the CPU restatement
+of TlmSolver's FP32 path, on identical inputs.
+
+Bar: bit-exact for pulses, stubs, probes, ports, Js and Huygens
+accumulators and the FP64 probe DFT (same operation order on both sides,
+no FMA contraction); FP64 energy within 1e-12 relative (the sum order of
+the reduction differs from the reference's chunked sum).
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from emsolver_amd import scenes, tlm
+from tests import helpers
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_scene(nx, ny, nz, seed, rho=0.0, pec_frac=0.05, diel=True, steps=40, random_state=True):
+    rng = np.random.default_rng(seed)
+    n = nx * ny * nz
+    mat = np.zeros(n, np.uint8)
+    nd = 4 if diel else 0
+    u = rng.random(n)
+    mat[u < pec_frac] = scenes.MAT_PEC
+    if diel:
+        ids = rng.integers(scenes.MAT_DIEL0, scenes.MAT_DIEL0 + nd, n).astype(np.uint8)
+        sel = (u >= pec_frac) & (u < pec_frac + 0.4)
+        mat[sel] = ids[sel]
+    epsr = rng.uniform(1.0, 10.0, nd).astype(np.float32)
+    sigma = rng.uniform(0.0, 0.05, nd).astype(np.float32)
+    sc = scenes.Scene(nx, ny, nz, mat, epsr, sigma, rho=rho, total_steps=steps, settle_steps=5,
+                      name=f"random_{nx}x{ny}x{nz}_s{seed}")
+    air = np.flatnonzero(mat == 0)
+    sc.sources = [(int(air[len(air) // 2]), 2, 1.0), (int(air[len(air) // 3]), 0, -0.7),
+                  (int(air[len(air) // 2]), 1, 0.25)]
+    sc.probes = np.array(rng.choice(n, 5, replace=False), np.int64)
+    sc.dft_freqs = np.array([1e8, 5e8, 1e9])
+    if random_state:
+        V0 = rng.standard_normal(12 * n).astype(np.float32)
+        V0.reshape(n, 12)[mat == scenes.MAT_PEC] = 0.0
+        sc.V0 = V0
+        if diel:
+            Vs0 = rng.standard_normal(3 * n).astype(np.float32)
+            Vs0.reshape(n, 3)[mat < scenes.MAT_DIEL0] = 0.0
+            sc.Vs0 = Vs0
+    return sc
+
+
+def _check(sc, steps=None):
+    print(f"[parity] {sc.name} rho={sc.rho} steps={steps or sc.total_steps}", flush=True)
+    g = tlm.run_scene(sc, steps=steps)
+    o = helpers.oracle_run(sc, steps=steps)
+    helpers.compare_results(g, o, sc)
+    return g, o
+
+
+def test_device_present():
+    assert tlm.device_count() >= 1
+    print(tlm.device_name(0))
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8), (13, 7, 9), (33, 5, 6), (64, 3, 17)])
+@pytest.mark.parametrize("rho", [0.0, -1.0, 0.37])
+def test_random_materials_bitexact(shape, rho):
+    sc = _random_scene(*shape, seed=sum(shape), rho=rho, steps=24)
+    _check(sc)
+
+
+def test_no_dielectric_bitexact():
+    sc = _random_scene(20, 12, 16, seed=5, diel=False, steps=30)
+    _check(sc)
+
+
+def test_free_space_cube_short():
+    sc = scenes.free_space_cube(n=32, steps=120, probe_offset=6)
+    _check(sc)
+
+
+def test_dielectric_boxes_small():
+    sc = scenes.dielectric_boxes(n=40, nboxes=5, steps=150, nprobes=6, nfreq=8, seed=2)
+    _check(sc)
+
+
+def test_selftest_dipole_ports_huygens():
+    sc = scenes.selftest_dipole(scenes.WF_GAUSS_SINE, extra_steps=0)
+    sc = sc.with_steps(min(sc.total_steps, sc.settle_steps + 80))
+    _check(sc)
+
+
+def test_selftest_dipole_with_sphere():
+    sc = scenes.selftest_dipole(scenes.WF_CW_RAMPED, extra_steps=0, diel_sphere=True)
+    sc = sc.with_steps(min(sc.total_steps, sc.settle_steps + 40))
+    _check(sc)
+
+
+def test_wide_rows_two_waves():
+    # nx = 300: one row spans 75 quads; exercises multi-wave rows
+    sc = _random_scene(300, 6, 5, seed=11, rho=-1.0, steps=12)
+    _check(sc)
+
+
+def test_free_space_64_full_run_tolerances():
+    """North-star tolerances on the 64^3 free-space case: probe max rel err
+    <= 1e-5 and L2-normalised snapshot err <= 1e-6 (bit-exact in practice)."""
+    sc = scenes.free_space_cube(n=64, steps=1000)
+    g = tlm.run_scene(sc)
+    o = helpers.oracle_run(sc)
+    pe = np.abs(g.probes - o.probes).max() / np.abs(o.probes).max()
+    se = np.linalg.norm(g.V - o.V) / np.linalg.norm(o.V)
+    print(f"probe rel err {pe:.3e}  snapshot rel L2 {se:.3e}")
+    assert pe <= 1e-5 and se <= 1e-6
+    helpers.compare_results(g, o, sc)
