@@ -1,0 +1,217 @@
+#!/usr/bin/env python3
+"""Throughput of the SCN TLM time step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload (one "step" = one TLM time step over the whole grid): the 256^3
+scene with 8 seeded dielectric boxes (eps_r in [1,10], sigma in [0,0.05]
+S/m), a PEC ground plane at z = 0, a centre Ez Gaussian-derivative soft
+source, 16 seeded E probes and a running FP64 probe DFT at 32 frequencies,
+with the PEC surface-current and energy monitors on, as the reference runs
+them. Inputs are resident in HBM before the timed region.
+
+Prints ONE JSON line. `value` is Gcell-updates/s of the timed K steps
+(wall clock between two stream synchronisations). `roofline` prices the
+fused scatter/connect kernel: algorithmic bytes per launch
+(cells * (97 + 24 * f_diel) B for FP32 pulses) over its average launch
+time, measured with HIP events on the context stream inside the timed
+region, against the 8 TB/s HBM peak. `cpu_baseline` times the CPU
+restatement of the reference algorithm (oracle/, the reference's threading
+rule min(16, hw-1)) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "TLM Gcell-updates/s & % HBM roofline; served mixed-size p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def alg_bytes_per_cell(scene, pulse_bytes=4, stub_bytes=4):
+    """B_alg = 2*12*s_p + 1 + 2*3*s_s*f_diel (material id 1 B)."""
+    return 2 * 12 * pulse_bytes + 1 + 2 * 3 * stub_bytes * scene.diel_fraction()
+
+
+def read_traffic(workload):
+    """HBM bytes per launch of the fused kernel from the committed rocprofv3
+    PMC summary of this workload, if present (line 'traffic_bytes_per_launch:')."""
+    path = os.path.join(ROOT, "profiles", f"pmc_k_step_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read())
+    return float(m.group(1)) if m else None
+
+
+def gpu_run(scene, warmup, steps, tag):
+    from emsolver_amd import tlm
+    sc = scene.with_steps(warmup + steps)
+    t_setup = time.perf_counter()
+    dev = tlm.open_scene(sc)
+    log(f"[bench] {tag}: set-up {time.perf_counter() - t_setup:.1f}s, shape {dev.launch_shape()}")
+    if warmup:
+        dev.run(0, warmup)
+    dev.sync()
+    t0 = time.perf_counter()
+    _, ms_kernel = dev.time_steps(warmup, steps, kernel=True)
+    dev.sync()
+    wall = time.perf_counter() - t0
+    cells = sc.ncell
+    out = {
+        "wall_s": wall,
+        "gcells": cells * steps / wall / 1e9,
+        "ms_per_step": wall * 1e3 / steps,
+        "kernel_ms_avg": ms_kernel / steps,
+    }
+    b = alg_bytes_per_cell(sc)
+    out["alg_bytes_per_cell"] = b
+    out["achieved_gbs"] = cells * b / (out["kernel_ms_avg"] * 1e-3) / 1e9
+    # a touch of the results, so a broken run cannot pass silently
+    if dev.nprobe:
+        e = dev.read_probes(0, warmup + steps)
+        out["probe_absmax"] = float(np.abs(e).max())
+    dev.close()
+    return out
+
+
+def cpu_baseline(scene, budget_s=15.0):
+    from tests import helpers
+    o = helpers.oracle_open(scene.with_steps(10**6))
+    t0 = time.perf_counter()
+    o.run_steps(2)
+    per = (time.perf_counter() - t0) / 2
+    n = int(max(2, min(200, budget_s / max(per, 1e-6))))
+    t0 = time.perf_counter()
+    o.run_steps(n)
+    dt = time.perf_counter() - t0
+    from oracle.oracle_tlm import load_library
+    threads = load_library().ot_default_threads()
+    o.close()
+    return {
+        "value": scene.ncell * n / dt / 1e9,
+        "unit": "Gcell-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{scene.name}: steps 2..{n + 1} ({n} steps, {dt:.1f}s) of the same scene with "
+                  f"all monitors, CPU restatement of TlmSolver's step with its ThreadPool rule "
+                  f"min(16, hw-1) = {threads} threads; host {os.cpu_count()} logical CPUs",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=256, help="grid edge of the dielectric-box scene")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-extra", action="store_true", help="skip the 512^3 free-space line")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU sharding of the grid is a "
+              "design note (z-slab halo exchange), not an implemented path. Use --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from emsolver_amd import scenes, tlm
+    if tlm.device_count() < 1:
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    dev_name = tlm.device_name(0)
+    log(f"[bench] device {dev_name}")
+
+    n = args.size
+    workload = f"diel_boxes_{n}"
+    scene = scenes.dielectric_boxes(n=n, nboxes=8, steps=args.warmup + args.steps, nprobes=16,
+                                    nfreq=32, seed=1)
+    log(f"[bench] {workload}: {scene.ncell} cells, f_diel {scene.diel_fraction():.4f}, "
+        f"warmup {args.warmup}, steps {args.steps}")
+    main_run = gpu_run(scene, args.warmup, args.steps, workload)
+    log(f"[bench] {workload}: {main_run['gcells']:.2f} Gcell/s wall, kernel "
+        f"{main_run['kernel_ms_avg']:.3f} ms, {main_run['achieved_gbs']:.0f} GB/s algorithmic")
+
+    extra = {}
+    lib = tlm._native.load()
+    import ctypes as C
+    gbps, msc = C.c_float(0), C.c_float(0)
+    if lib.tlm_copy_bandwidth(0, 2 << 30, 20, C.byref(gbps), C.byref(msc)) == 0:
+        extra["device_copy_gbs"] = round(gbps.value, 1)
+        log(f"[bench] device copy {gbps.value:.0f} GB/s")
+    if not args.no_extra:
+        big = scenes.free_space_cube(n=512, steps=200, probe_offset=10)
+        r = gpu_run(big, 10, 200, "free_space_512")
+        extra["free_space_512_f32"] = {
+            "gcell_s": round(r["gcells"], 3),
+            "kernel_ms_avg": round(r["kernel_ms_avg"], 4),
+            "achieved_gbs": round(r["achieved_gbs"], 1),
+            "roofline_frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
+            "alg_bytes_per_cell": r["alg_bytes_per_cell"],
+            "steps": 200,
+        }
+        log(f"[bench] free_space_512: {r['gcells']:.2f} Gcell/s, kernel {r['kernel_ms_avg']:.3f} ms, "
+            f"{r['achieved_gbs']:.0f} GB/s = {r['achieved_gbs'] / HBM_PEAK_GBS:.3f} of peak")
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] CPU baseline on {workload} (budget {args.cpu_budget}s)")
+        cpu = cpu_baseline(scene, args.cpu_budget)
+        log(f"[bench] CPU baseline {cpu['value']:.4f} Gcell/s on {cpu['cores']} threads")
+
+    ach = main_run["achieved_gbs"]
+    line = {
+        "metric": METRIC,
+        "value": round(main_run["gcells"], 4),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(main_run["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{workload}: {n}^3 SCN TLM, 8 seeded dielectric boxes "
+                        "(eps_r U[1,10], sigma U[0,0.05] S/m), PEC ground plane z=0, centre Ez "
+                        "Gaussian-derivative soft source (w=20, delay 60 steps), 16 E probes, "
+                        "FP64 probe DFT at 32 frequencies, Js + energy monitors",
+            "grid": [n, n, n],
+            "diel_fraction": round(scene.diel_fraction(), 5),
+            "device": dev_name,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(ach, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": read_traffic(workload),
+            "kernel": "tlm::k_step (fused scatter+connect)",
+            "kernel_ms_avg": round(main_run["kernel_ms_avg"], 5),
+            "alg_bytes_per_launch": int(round(scene.ncell * main_run["alg_bytes_per_cell"])),
+        },
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -56,6 +56,7 @@ SIGNATURES = {
     "tlm_time_steps": (C.c_int, [P, C.c_int, C.c_int, fp, fp]),
     "tlm_launch_shape": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                    C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "tlm_copy_bandwidth": (C.c_int, [C.c_int, C.c_size_t, C.c_int, fp, fp]),
     "tlm_scatter_unitarity_error": (C.c_float, [C.c_float]),
     "tlm_build_pec_faces": (C.c_int64, [C.c_int, C.c_int, C.c_int, u8p, i64p, i32p, i32p,
                                         C.c_int64]),

@@ -34,6 +34,7 @@ struct tlm_ctx
     int dev = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evk0 = nullptr, evk1 = nullptr;
+    std::vector<hipEvent_t> kev; // per-launch timing events (2 per step), created on demand
     int nx = 0, ny = 0, nz = 0, px = 0;
     long long P = 0;
     float dl = 0.f, rho = 0.f;
@@ -222,7 +223,16 @@ int queue_steps(tlm_ctx *c, int step0, int nsteps, bool time_kernel, float *ms_k
         return fail(c, TLM_ERR_STATE, "step range exceeds the schedule set by tlm_set_schedule");
     HIPCHK(c, hipSetDevice(c->dev));
     const int nmon = c->nf + c->nh + c->np + c->npr;
-    float kern_ms = 0.f;
+    if (time_kernel)
+    {
+        // events bracket every fused-kernel launch; no host wait inside the loop
+        while ((int)c->kev.size() < 2 * nsteps)
+        {
+            hipEvent_t ev;
+            HIPCHK(c, hipEventCreate(&ev));
+            c->kev.push_back(ev);
+        }
+    }
     for (int n = step0; n < step0 + nsteps; ++n)
     {
         if (c->nuniq > 0)
@@ -263,16 +273,10 @@ int queue_steps(tlm_ctx *c, int step0, int nsteps, bool time_kernel, float *ms_k
             hipLaunchKernelGGL(tlm::k_monitor, dim3(g), dim3(b), 0, c->stream, m);
         }
         if (time_kernel)
-            HIPCHK(c, hipEventRecord(c->evk0, c->stream));
+            HIPCHK(c, hipEventRecord(c->kev[2 * (n - step0)], c->stream));
         launch_step(c, n);
         if (time_kernel)
-        {
-            HIPCHK(c, hipEventRecord(c->evk1, c->stream));
-            HIPCHK(c, hipEventSynchronize(c->evk1));
-            float ms = 0.f;
-            HIPCHK(c, hipEventElapsedTime(&ms, c->evk0, c->evk1));
-            kern_ms += ms;
-        }
+            HIPCHK(c, hipEventRecord(c->kev[2 * (n - step0) + 1], c->stream));
         if (c->every > 0 && (n % c->every) == 0)
         {
             const int idx = n / c->every;
@@ -297,8 +301,18 @@ int queue_steps(tlm_ctx *c, int step0, int nsteps, bool time_kernel, float *ms_k
         c->dft_done = step0 + nsteps;
     }
     c->steps_run_hi = std::max(c->steps_run_hi, step0 + nsteps);
-    if (ms_kernel)
+    if (time_kernel && ms_kernel)
+    {
+        float kern_ms = 0.f;
+        HIPCHK(c, hipEventSynchronize(c->kev[2 * nsteps - 1]));
+        for (int s = 0; s < nsteps; ++s)
+        {
+            float ms = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&ms, c->kev[2 * s], c->kev[2 * s + 1]));
+            kern_ms += ms;
+        }
         *ms_kernel = kern_ms;
+    }
     HIPCHK(c, hipGetLastError());
     return TLM_OK;
 }
@@ -441,6 +455,8 @@ void tlm_destroy(tlm_ctx *c)
     for (hipEvent_t ev : {c->ev0, c->ev1, c->evk0, c->evk1})
         if (ev)
             (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : c->kev)
+        (void)hipEventDestroy(ev);
     if (c->stream)
         (void)hipStreamDestroy(c->stream);
     delete c;
@@ -911,6 +927,50 @@ int tlm_read_energy(tlm_ctx *c, int32_t *steps, double *vals, int cap)
         }
     }
     return cnt;
+}
+
+int tlm_copy_bandwidth(int device, size_t bytes, int iters, float *gbps, float *ms_per_copy)
+{
+    if (bytes < 16 || iters <= 0)
+        return TLM_ERR_ARG;
+    if (tlm_device_count() <= device || device < 0)
+        return TLM_ERR_NO_DEVICE;
+    if (hipSetDevice(device) != hipSuccess)
+        return TLM_ERR_HIP;
+    const size_t n4 = bytes / 16;
+    float4 *src = nullptr, *dst = nullptr;
+    if (hipMalloc((void **)&src, n4 * 16) != hipSuccess)
+        return TLM_ERR_NOMEM;
+    if (hipMalloc((void **)&dst, n4 * 16) != hipSuccess)
+    {
+        (void)hipFree(src);
+        return TLM_ERR_NOMEM;
+    }
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipMemset(src, 0, n4 * 16);
+    const int b = 256;
+    const unsigned g = (unsigned)std::min<size_t>((n4 + b - 1) / b, 1u << 20);
+    hipLaunchKernelGGL(tlm::k_copy, dim3(g), dim3(b), 0, 0, src, dst, (long long)n4);
+    (void)hipEventRecord(e0, 0);
+    for (int i = 0; i < iters; ++i)
+        hipLaunchKernelGGL(tlm::k_copy, dim3(g), dim3(b), 0, 0, src, dst, (long long)n4);
+    (void)hipEventRecord(e1, 0);
+    hipError_t e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    if (e != hipSuccess)
+        return TLM_ERR_HIP;
+    if (ms_per_copy)
+        *ms_per_copy = ms / iters;
+    if (gbps)
+        *gbps = (float)(2.0 * n4 * 16 * iters / (ms * 1e-3) / 1e9);
+    return TLM_OK;
 }
 
 float tlm_scatter_unitarity_error(float epsr)

@@ -620,4 +620,13 @@ __global__ void k_scatter_columns(float yh, float den, int diel, float *S /*[15]
         S[(12 + r) * 15 + c] = vs[r];
 }
 
+// Plain grid-stride copy: the reference kernel for the measured device-copy
+// bandwidth and for scaling the FETCH_SIZE / WRITE_SIZE counters.
+__global__ void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, long long n4)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+         i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 } // namespace tlm

@@ -156,6 +156,12 @@ int tlm_read_energy(tlm_ctx *ctx, int32_t *steps, double *vals, int cap);
  * is the summed duration of the fused scatter/connect launches alone. */
 int tlm_time_steps(tlm_ctx *ctx, int step0, int nsteps, float *ms_total, float *ms_kernel);
 
+/* Device-copy bandwidth: copies `bytes` between two device buffers `iters`
+ * times with a plain grid-stride float4 kernel (k_copy) and reports
+ * read+write GB/s and the average time of one copy. The roofline reference
+ * for the HBM-bound step; no counterpart in the reference. */
+int tlm_copy_bandwidth(int device, size_t bytes, int iters, float *gbps, float *ms_per_copy);
+
 /* Launch shape chosen for the fused kernel: rows per workgroup, planes per
  * workgroup, workgroup count. For reporting. */
 int tlm_launch_shape(const tlm_ctx *ctx, int *rows_per_group, int *planes_per_group,
