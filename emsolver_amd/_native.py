@@ -28,6 +28,11 @@ class GridDesc(C.Structure):
                 ("reserved", C.c_int32)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("kernel", C.c_int32), ("planes", C.c_int32), ("rows", C.c_int32),
+                ("nontemporal", C.c_int32)]
+
+
 # Every symbol declared in include/GpuTlmHip.h with its ctypes signature.
 SIGNATURES = {
     "tlm_device_count": (C.c_int, []),
@@ -56,6 +61,8 @@ SIGNATURES = {
     "tlm_time_steps": (C.c_int, [P, C.c_int, C.c_int, fp, fp]),
     "tlm_launch_shape": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                    C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "tlm_set_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
+    "tlm_get_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
     "tlm_copy_bandwidth": (C.c_int, [C.c_int, C.c_size_t, C.c_int, fp, fp]),
     "tlm_scatter_unitarity_error": (C.c_float, [C.c_float]),
     "tlm_build_pec_faces": (C.c_int64, [C.c_int, C.c_int, C.c_int, u8p, i64p, i32p, i32p,

@@ -40,6 +40,9 @@ struct tlm_ctx
     float dl = 0.f, rho = 0.f;
     // launch shape of the fused kernel
     int Q = 0, R = 0, zc = 0, gx = 0, gy = 0, threads = 0;
+    tlm_tuning tune = {1, 0, 0, 0};
+    int wq = 0, wr = 0, wseg = 0, wtasks = 0, wzc = 0, wnzc = 0, wblocks = 0; // wave kernel
+    size_t epart_cap = 0;
     // state
     float *A = nullptr, *B = nullptr, *Vs = nullptr;
     uint8_t *mat = nullptr;
@@ -158,24 +161,89 @@ bool cell_ok(const tlm_ctx *c, long long cell)
     return cell >= 0 && cell < (long long)c->nx * c->ny * c->nz;
 }
 
-// Launch shape: each workgroup holds R whole rows (Q quads each) and
-// marches over zc planes. zc is chosen so the grid has enough workgroups to
-// fill 256 CUs several times over.
+// Launch shapes.
+// Kernel 0 (workgroup rows): each workgroup holds R whole rows (Q quads
+// each) and marches over zc planes; zc is chosen so the grid has enough
+// workgroups to fill 256 CUs several times over.
+// Kernel 1 (independent waves): a wave holds rw row-strips of qw quads and
+// marches over wzc planes; by default about 16K wave tasks per launch.
 void choose_shape(tlm_ctx *c)
 {
     c->Q = c->px / 4;
-    c->R = std::max(1, tlm::kMaxThreads / c->Q);
-    c->R = std::min(c->R, c->ny);
+    c->R = c->tune.rows > 0 ? c->tune.rows : std::max(1, tlm::kMaxThreads / c->Q);
+    c->R = std::max(1, std::min(std::min(c->R, c->ny), tlm::kMaxThreads / c->Q));
     c->threads = ((c->Q * c->R + 63) / 64) * 64;
     c->gx = (c->ny + c->R - 1) / c->R;
     const int target = 2048;
     int nchunks = std::max(1, std::min(c->nz, (target + c->gx - 1) / c->gx));
     c->zc = (c->nz + nchunks - 1) / nchunks;
+    if (c->tune.kernel == 0 && c->tune.planes > 0)
+        c->zc = std::min(c->tune.planes, c->nz);
     c->gy = (c->nz + c->zc - 1) / c->zc;
+
+    c->wq = std::min(c->Q, 64);
+    c->wr = 64 / c->wq;
+    c->wseg = (c->Q + c->wq - 1) / c->wq;
+    c->wtasks = ((c->ny + c->wr - 1) / c->wr) * c->wseg;
+    if (c->tune.kernel == 1 && c->tune.planes > 0)
+        c->wzc = std::min(c->tune.planes, c->nz);
+    else
+    {
+        const long long target_waves = 16384;
+        int nch = (int)std::max(1LL, std::min<long long>(c->nz, target_waves / c->wtasks));
+        c->wzc = (c->nz + nch - 1) / nch;
+    }
+    c->wnzc = (c->nz + c->wzc - 1) / c->wzc;
+    c->wblocks = (int)(((long long)c->wtasks * c->wnzc + 3) / 4);
+}
+
+size_t energy_parts(const tlm_ctx *c)
+{
+    return c->tune.kernel == 1 ? (size_t)c->wtasks * c->wnzc : (size_t)c->gx * c->gy;
+}
+
+template <bool S, bool E>
+void launch_wave(tlm_ctx *c, const tlm::WaveArgs &a)
+{
+    if (c->tune.nontemporal)
+        hipLaunchKernelGGL((tlm::k_step_wave<S, E, true>), dim3(c->wblocks), dim3(256), 0,
+                           c->stream, a);
+    else
+        hipLaunchKernelGGL((tlm::k_step_wave<S, E, false>), dim3(c->wblocks), dim3(256), 0,
+                           c->stream, a);
 }
 
 int launch_step(tlm_ctx *c, int n)
 {
+    const bool energy = c->every > 0 && (n % c->every) == 0;
+    if (c->tune.kernel == 1)
+    {
+        tlm::WaveArgs w;
+        w.A = c->A;
+        w.B = c->B;
+        w.Vs = c->Vs;
+        w.mat = c->mat;
+        w.props = c->props;
+        w.epart = c->epart;
+        w.nx = c->nx;
+        w.ny = c->ny;
+        w.nz = c->nz;
+        w.px = c->px;
+        w.Q = c->Q;
+        w.qw = c->wq;
+        w.rw = c->wr;
+        w.nseg = c->wseg;
+        w.tasks_xy = c->wtasks;
+        w.zc = c->wzc;
+        w.nzc = c->wnzc;
+        w.P = c->P;
+        w.rho = c->rho;
+        if (c->stubs)
+            energy ? launch_wave<true, true>(c, w) : launch_wave<true, false>(c, w);
+        else
+            energy ? launch_wave<false, true>(c, w) : launch_wave<false, false>(c, w);
+        return TLM_OK;
+    }
     tlm::StepArgs a;
     a.A = c->A;
     a.B = c->B;
@@ -192,7 +260,6 @@ int launch_step(tlm_ctx *c, int n)
     a.zc = c->zc;
     a.P = c->P;
     a.rho = c->rho;
-    const bool energy = c->every > 0 && (n % c->every) == 0;
     dim3 grid(c->gx, c->gy), block(c->threads);
     if (c->stubs)
     {
@@ -281,7 +348,7 @@ int queue_steps(tlm_ctx *c, int step0, int nsteps, bool time_kernel, float *ms_k
         {
             const int idx = n / c->every;
             hipLaunchKernelGGL(tlm::k_energy_reduce, dim3(1), dim3(256), 0, c->stream, c->epart,
-                               c->gx * c->gy, c->energy + idx);
+                               (int)energy_parts(c), c->energy + idx);
             if ((int)c->energy_steps.size() <= idx)
                 c->energy_steps.resize(idx + 1, -1);
             c->energy_steps[idx] = n;
@@ -434,9 +501,9 @@ int tlm_create(const tlm_grid_desc *d, tlm_ctx **out)
         if ((e = hipEventCreate(ev)) != hipSuccess)
             return bail(hip_fail(c, e, "hipEventCreate"));
     int rc;
+    c->epart_cap = std::max((size_t)c->gx * c->gy, (size_t)c->wtasks * c->wnzc);
     if ((rc = dalloc(c, &c->A, (size_t)c->P * 12)) || (rc = dalloc(c, &c->B, (size_t)c->P * 12)) ||
-        (rc = dalloc(c, &c->mat, (size_t)c->P)) ||
-        (rc = dalloc(c, &c->epart, (size_t)c->gx * c->gy)))
+        (rc = dalloc(c, &c->mat, (size_t)c->P)) || (rc = dalloc(c, &c->epart, c->epart_cap)))
         return bail(rc);
     *out = c;
     return TLM_OK;
@@ -804,14 +871,43 @@ int tlm_launch_shape(const tlm_ctx *c, int *rows, int *planes, int *groups, int 
 {
     if (!c)
         return TLM_ERR_ARG;
+    const bool w = c->tune.kernel == 1;
     if (rows)
-        *rows = c->R;
+        *rows = w ? c->wr : c->R;
     if (planes)
-        *planes = c->zc;
+        *planes = w ? c->wzc : c->zc;
     if (groups)
-        *groups = c->gx * c->gy;
+        *groups = w ? c->wblocks : c->gx * c->gy;
     if (threads)
-        *threads = c->threads;
+        *threads = w ? 256 : c->threads;
+    return TLM_OK;
+}
+
+int tlm_set_tuning(tlm_ctx *c, const tlm_tuning *t)
+{
+    if (!c || !t || t->kernel < 0 || t->kernel > 1 || t->planes < 0 || t->rows < 0 ||
+        t->nontemporal < 0 || t->nontemporal > 1)
+        return fail(c, TLM_ERR_ARG, "tlm_set_tuning: bad arguments");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->tune = *t;
+    choose_shape(c);
+    const size_t need = energy_parts(c);
+    if (need > c->epart_cap)
+    {
+        dfree(c, c->epart);
+        int rc = dalloc(c, &c->epart, need);
+        if (rc)
+            return rc;
+        c->epart_cap = need;
+    }
+    return TLM_OK;
+}
+
+int tlm_get_tuning(const tlm_ctx *c, tlm_tuning *t)
+{
+    if (!c || !t)
+        return TLM_ERR_ARG;
+    *t = c->tune;
     return TLM_OK;
 }
 

@@ -375,6 +375,347 @@ __global__ void __launch_bounds__(kMaxThreads) k_step(StepArgs a)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Wave-independent variant of the fused step: no LDS and no barrier.
+//
+// Each wavefront owns a strip of rows (qw quads per row-strip, rw strips per
+// wave) and marches over zc planes. x neighbours inside the wave are read
+// with lane shuffles; where a row is split between two waves (rows wider
+// than 256 cells) the two seam cells push their x pulse into each other's
+// slot, exactly like the y and z-chunk links. Incident pulses are not kept
+// live after the scatter: the rare PEC cells reload their (unchanged) value.
+// ---------------------------------------------------------------------------
+struct WaveArgs
+{
+    const float *A;
+    float *B;
+    float *Vs;
+    const uint8_t *mat;
+    const float4 *props;
+    double *epart; // energy partial per wave
+    int nx, ny, nz, px, Q;
+    int qw;        // quads per wave row-strip: min(Q, 64)
+    int rw;        // row-strips per wave: 64 / qw
+    int nseg;      // strips per row: ceil(Q / qw)
+    int tasks_xy;  // wave tasks per plane range: ceil(ny / rw) * nseg
+    int zc, nzc;   // planes per task, number of z chunks
+    long long P;
+    float rho;
+};
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+template <bool NT> __device__ __forceinline__ void st4x(float *p, const float (&r)[4])
+{
+    if (NT)
+    {
+        f4v x = {r[0], r[1], r[2], r[3]};
+        __builtin_nontemporal_store(x, reinterpret_cast<f4v *>(p));
+    }
+    else
+        st4(p, r);
+}
+template <bool NT> __device__ __forceinline__ void st4x_masked(float *p, const float (&r)[4],
+                                                               unsigned mask)
+{
+    if (mask == 0xFu)
+        st4x<NT>(p, r);
+    else if (mask)
+    {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            if (mask & (1u << e))
+                p[e] = r[e];
+    }
+}
+__device__ __forceinline__ void ld4_if(const float *p, float (&r)[4], bool cond)
+{
+    if (cond)
+        ld4(p, r);
+    else
+        r[0] = r[1] = r[2] = r[3] = 0.0f;
+}
+__device__ __forceinline__ bool byte_is_pec(uint32_t m, int e)
+{
+    return ((m >> (8 * e)) & 0xFFu) == 1u;
+}
+
+template <bool STUBS, bool ENERGY, bool NT>
+__global__ void __launch_bounds__(kMaxThreads) k_step_wave(WaveArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int tx = gw % a.tasks_xy, tz = gw / a.tasks_xy;
+    if (tz >= a.nzc)
+        return; // whole wave
+    const int r = lane / a.qw, ql = lane - r * a.qw;
+    const int jg = tx / a.nseg, s = tx - jg * a.nseg;
+    const int j = jg * a.rw + r, q = s * a.qw + ql;
+    const bool act = (r < a.rw) && (j < a.ny) && (q < a.Q);
+    const bool hasL = ql > 0, hasR = (ql < a.qw - 1) && (q < a.Q - 1);
+    const bool seamL = act && ql == 0 && q > 0;
+    const bool seamR = act && ql == a.qw - 1 && q < a.Q - 1;
+    const int k0 = tz * a.zc, k1 = min(a.nz, k0 + a.zc);
+    const int i0 = 4 * q;
+    const long long sy = a.px, sz = (long long)a.px * a.ny, P = a.P;
+    const float rho = a.rho;
+
+    float zpo[2][4]; // previous plane's reflected z+ pulses
+    uint32_t mprev = 0x01010101u;
+    double esum = 0.0;
+
+    for (int k = k0; k < k1; ++k)
+    {
+        const long long c = ((long long)k * a.ny + j) * a.px + i0;
+        uint32_t m = 0x01010101u;
+        float o[12][4];
+        if (act)
+        {
+            m = *reinterpret_cast<const uint32_t *>(a.mat + c);
+            float v[12][4];
+#pragma unroll
+            for (int p = 0; p < 12; ++p)
+                ld4(a.A + p * P + c, v[p]);
+            if (ENERGY)
+            {
+#pragma unroll
+                for (int p = 0; p < 12; ++p)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        esum += (double)v[p][e] * (double)v[p][e];
+            }
+            const unsigned dm = STUBS ? diel_bits(m) : 0u;
+            float vs[3][4];
+            if (STUBS && dm)
+            {
+#pragma unroll
+                for (int t = 0; t < 3; ++t)
+                    ld4(a.Vs + t * P + c, vs[t]);
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+            {
+                float vin[12], vout[12];
+#pragma unroll
+                for (int p = 0; p < 12; ++p)
+                    vin[p] = v[p][e];
+                if (STUBS && (dm & (1u << e)))
+                {
+                    float4 pr = a.props[((m >> (8 * e)) & 0xFFu) - 2u];
+                    float st[3] = {vs[0][e], vs[1][e], vs[2][e]};
+                    scn::scatter_diel(vin, st, pr.x, pr.z, vout);
+                    vs[0][e] = st[0];
+                    vs[1][e] = st[1];
+                    vs[2][e] = st[2];
+                }
+                else
+                    scn::scatter_air(vin, vout);
+#pragma unroll
+                for (int p = 0; p < 12; ++p)
+                    o[p][e] = vout[p];
+            }
+            if (STUBS && dm)
+            {
+#pragma unroll
+                for (int t = 0; t < 3; ++t)
+                    st4(a.Vs + t * P + c, vs[t]);
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int p = 0; p < 12; ++p)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    o[p][e] = 0.0f;
+        }
+        // x neighbours inside the wave (every lane takes part in the shuffles)
+        const float Lz = __shfl_up(o[2][3], 1, 64), Lw = __shfl_up(o[3][3], 1, 64);
+        const float Rx = __shfl_down(o[0][0], 1, 64), Ry = __shfl_down(o[1][0], 1, 64);
+        const uint32_t mL = __shfl_up(m, 1, 64), mR = __shfl_down(m, 1, 64);
+        if (!act)
+            continue;
+        const unsigned pm = pec_bits(m);
+
+        // ---- x faces (ports 0,1 = x-, 2,3 = x+) ----
+        {
+            bool pecL = true, pecR = true;
+            if (hasL)
+                pecL = byte_is_pec(mL, 3);
+            else if (seamL)
+                pecL = a.mat[c - 1] == 1;
+            if (hasR)
+                pecR = byte_is_pec(mR, 0);
+            else if (seamR)
+                pecR = a.mat[c + 4] == 1;
+            const bool pec0 = pm & 1u, pec3 = (pm >> 3) & 1u;
+            const unsigned maskM = seamL ? (0xEu | ((pec0 || pecL) ? 1u : 0u)) : 0xFu;
+            const unsigned maskP = seamR ? (0x7u | ((pec3 || pecR) ? 8u : 0u)) : 0xFu;
+#pragma unroll
+            for (int pol = 0; pol < 2; ++pol)
+            {
+                float vm[4], vp[4], rm[4], rp[4];
+                ld4_if(a.A + (0 + pol) * P + c, vm, pm != 0u);
+                ld4_if(a.A + (2 + pol) * P + c, vp, pm != 0u);
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                {
+                    const int i = i0 + e;
+                    const bool pec = (pm >> e) & 1u;
+                    {
+                        const bool bnd = (i == 0);
+                        const float nbv = (e == 0) ? (pol ? Lw : Lz) : o[2 + pol][e - 1];
+                        const bool nbp = (e == 0) ? pecL : (((pm >> (e - 1)) & 1u) != 0u);
+                        const float oo = o[0 + pol][e];
+                        rm[e] = pec ? (bnd ? rho * vm[e] : vm[e])
+                                    : (bnd ? rho * oo : (nbp ? -oo : nbv));
+                    }
+                    {
+                        const bool bnd = (i == a.nx - 1);
+                        const float nbv = (e == 3) ? (pol ? Ry : Rx) : o[0 + pol][e + 1];
+                        const bool nbp = (e == 3) ? pecR : (((pm >> (e + 1)) & 1u) != 0u);
+                        const float oo = o[2 + pol][e];
+                        rp[e] = pec ? (bnd ? rho * vp[e] : vp[e])
+                                    : (bnd ? rho * oo : (nbp ? -oo : nbv));
+                    }
+                }
+                st4x_masked<NT>(a.B + (0 + pol) * P + c, rm, maskM);
+                st4x_masked<NT>(a.B + (2 + pol) * P + c, rp, maskP);
+                if (seamL && !pec0 && !pecL)
+                    a.B[(2 + pol) * P + c - 1] = o[0 + pol][0];
+                if (seamR && !pec3 && !pecR)
+                    a.B[(0 + pol) * P + c + 4] = o[2 + pol][3];
+            }
+        }
+
+        // ---- y faces (ports 4,5 = y-, 6,7 = y+): sender push ----
+        {
+            const bool bLo = (j == 0), bHi = (j == a.ny - 1);
+            const unsigned pLo = bLo ? 0u : pec_bits(*reinterpret_cast<const uint32_t *>(a.mat + c - sy));
+            const unsigned pHi = bHi ? 0u : pec_bits(*reinterpret_cast<const uint32_t *>(a.mat + c + sy));
+            const unsigned ownLo = bLo ? 0xFu : (pm | pLo);
+            const unsigned pushLo = bLo ? 0u : (~pm & ~pLo & 0xFu);
+            const unsigned ownHi = bHi ? 0xFu : (pm | pHi);
+            const unsigned pushHi = bHi ? 0u : (~pm & ~pHi & 0xFu);
+#pragma unroll
+            for (int pol = 0; pol < 2; ++pol)
+            {
+                float vl[4], vh[4], rl[4], rh[4];
+                ld4_if(a.A + (4 + pol) * P + c, vl, pm != 0u);
+                ld4_if(a.A + (6 + pol) * P + c, vh, pm != 0u);
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                {
+                    const bool pec = (pm >> e) & 1u;
+                    rl[e] = pec ? (bLo ? rho * vl[e] : vl[e]) : (bLo ? rho * o[4 + pol][e] : -o[4 + pol][e]);
+                    rh[e] = pec ? (bHi ? rho * vh[e] : vh[e]) : (bHi ? rho * o[6 + pol][e] : -o[6 + pol][e]);
+                }
+                st4x_masked<NT>(a.B + (4 + pol) * P + c, rl, ownLo);
+                st4x_masked<NT>(a.B + (6 + pol) * P + c - sy, o[4 + pol], pushLo);
+                st4x_masked<NT>(a.B + (6 + pol) * P + c, rh, ownHi);
+                st4x_masked<NT>(a.B + (4 + pol) * P + c + sy, o[6 + pol], pushHi);
+            }
+        }
+
+        // ---- z faces (ports 8,9 = z-, 10,11 = z+) ----
+        if (k > k0)
+        {
+            const unsigned pp = pec_bits(mprev);
+#pragma unroll
+            for (int pol = 0; pol < 2; ++pol)
+            {
+                float vz[4], vzp[4], rz[4], rzp[4];
+                ld4_if(a.A + (8 + pol) * P + c, vz, pm != 0u);
+                ld4_if(a.A + (10 + pol) * P + c - sz, vzp, pp != 0u);
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                {
+                    const bool pec = (pm >> e) & 1u, pecB = (pp >> e) & 1u;
+                    rz[e] = pec ? vz[e] : (pecB ? -o[8 + pol][e] : zpo[pol][e]);
+                    rzp[e] = pecB ? vzp[e] : (pec ? -zpo[pol][e] : o[8 + pol][e]);
+                }
+                st4x<NT>(a.B + (8 + pol) * P + c, rz);
+                st4x<NT>(a.B + (10 + pol) * P + c - sz, rzp);
+            }
+        }
+        else if (k == 0)
+        {
+#pragma unroll
+            for (int pol = 0; pol < 2; ++pol)
+            {
+                float vz[4], rz[4];
+                ld4_if(a.A + (8 + pol) * P + c, vz, pm != 0u);
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    rz[e] = ((pm >> e) & 1u) ? rho * vz[e] : rho * o[8 + pol][e];
+                st4x<NT>(a.B + (8 + pol) * P + c, rz);
+            }
+        }
+        else
+        {
+            const unsigned pLo = pec_bits(*reinterpret_cast<const uint32_t *>(a.mat + c - sz));
+            const unsigned own = pm | pLo, push = ~pm & ~pLo & 0xFu;
+#pragma unroll
+            for (int pol = 0; pol < 2; ++pol)
+            {
+                float vz[4], rz[4];
+                ld4_if(a.A + (8 + pol) * P + c, vz, pm != 0u);
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    rz[e] = ((pm >> e) & 1u) ? vz[e] : -o[8 + pol][e];
+                st4x_masked<NT>(a.B + (8 + pol) * P + c, rz, own);
+                st4x_masked<NT>(a.B + (10 + pol) * P + c - sz, o[8 + pol], push);
+            }
+        }
+        if (k == k1 - 1)
+        {
+            if (k == a.nz - 1)
+            {
+#pragma unroll
+                for (int pol = 0; pol < 2; ++pol)
+                {
+                    float vz[4], rz[4];
+                    ld4_if(a.A + (10 + pol) * P + c, vz, pm != 0u);
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        rz[e] = ((pm >> e) & 1u) ? rho * vz[e] : rho * o[10 + pol][e];
+                    st4x<NT>(a.B + (10 + pol) * P + c, rz);
+                }
+            }
+            else
+            {
+                const unsigned pHi = pec_bits(*reinterpret_cast<const uint32_t *>(a.mat + c + sz));
+                const unsigned own = pm | pHi, push = ~pm & ~pHi & 0xFu;
+#pragma unroll
+                for (int pol = 0; pol < 2; ++pol)
+                {
+                    float vz[4], rz[4];
+                    ld4_if(a.A + (10 + pol) * P + c, vz, pm != 0u);
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        rz[e] = ((pm >> e) & 1u) ? vz[e] : -o[10 + pol][e];
+                    st4x_masked<NT>(a.B + (10 + pol) * P + c, rz, own);
+                    st4x_masked<NT>(a.B + (8 + pol) * P + c + sz, o[10 + pol], push);
+                }
+            }
+        }
+#pragma unroll
+        for (int pol = 0; pol < 2; ++pol)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                zpo[pol][e] = o[10 + pol][e];
+        mprev = m;
+    }
+
+    if (ENERGY)
+    {
+        for (int off = 32; off > 0; off >>= 1)
+            esum += __shfl_down(esum, off, 64);
+        if (lane == 0)
+            a.epart[gw] = esum;
+    }
+}
+
 // Soft-source injection (TlmSolver.cpp:487-505). One thread per distinct
 // cell adds its sources in list order, so every slot sees the same
 // sequence of additions as the reference.

@@ -235,6 +235,16 @@ class TlmDevice:
                                           C.byref(ker) if kernel else None), "tlm_time_steps")
         return tot.value, (ker.value if kernel else None)
 
+    def set_tuning(self, kernel=1, planes=0, rows=0, nontemporal=0):
+        t = _native.Tuning(int(kernel), int(planes), int(rows), int(nontemporal))
+        self._chk(self.lib.tlm_set_tuning(self.h, C.byref(t)), "tlm_set_tuning")
+
+    def get_tuning(self):
+        t = _native.Tuning()
+        self._chk(self.lib.tlm_get_tuning(self.h, C.byref(t)), "tlm_get_tuning")
+        return {"kernel": t.kernel, "planes": t.planes, "rows": t.rows,
+                "nontemporal": t.nontemporal}
+
     def launch_shape(self):
         r, z, g, t = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         self._chk(self.lib.tlm_launch_shape(self.h, C.byref(r), C.byref(z), C.byref(g),

@@ -112,3 +112,33 @@ def test_free_space_64_full_run_tolerances():
     print(f"probe rel err {pe:.3e}  snapshot rel L2 {se:.3e}")
     assert pe <= 1e-5 and se <= 1e-6
     helpers.compare_results(g, o, sc)
+
+
+TUNINGS = [
+    {"kernel": 0, "planes": 0, "rows": 0, "nontemporal": 0},
+    {"kernel": 0, "planes": 3, "rows": 1, "nontemporal": 0},
+    {"kernel": 1, "planes": 0, "rows": 0, "nontemporal": 0},
+    {"kernel": 1, "planes": 1, "rows": 0, "nontemporal": 1},
+    {"kernel": 1, "planes": 5, "rows": 0, "nontemporal": 0},
+]
+
+
+def _run_tuned(sc, tuning, steps=None):
+    steps = sc.total_steps if steps is None else steps
+    with tlm.open_scene(sc) as dev:
+        dev.set_tuning(**tuning)
+        dev.run(0, steps)
+        dev.sync()
+        return tlm.collect(dev, sc, steps)
+
+
+@pytest.mark.parametrize("tuning", TUNINGS, ids=lambda t: "k{kernel}_z{planes}_r{rows}_nt{nontemporal}".format(**t))
+@pytest.mark.parametrize("shape", [(12, 9, 11), (300, 5, 7), (600, 3, 4), (1024, 2, 3)])
+def test_tunings_bitexact(shape, tuning):
+    """Every launch shape / kernel variant gives the oracle's bits, including
+    rows split across 2-4 waves (nx = 300, 600, 1024) where seam cells push."""
+    sc = _random_scene(*shape, seed=shape[0] + 7, rho=0.37, steps=10)
+    print(f"[parity] {sc.name} tuning {tuning}", flush=True)
+    g = _run_tuned(sc, tuning)
+    o = helpers.oracle_run(sc)
+    helpers.compare_results(g, o, sc)
