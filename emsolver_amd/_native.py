@@ -29,7 +29,7 @@ class GridDesc(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("kernel", C.c_int32), ("planes", C.c_int32), ("rows", C.c_int32),
+    _fields_ = [("kernel", C.c_int32), ("planes", C.c_int32), ("block_waves", C.c_int32),
                 ("nontemporal", C.c_int32)]
 
 

@@ -35,13 +35,12 @@ struct tlm_ctx
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evk0 = nullptr, evk1 = nullptr;
     std::vector<hipEvent_t> kev; // per-launch timing events (2 per step), created on demand
-    int nx = 0, ny = 0, nz = 0, px = 0;
-    long long P = 0;
+    int nx = 0, ny = 0, nz = 0;
+    long long nxy = 0, Pp = 0, P = 0; // slab pitch roundup(nx*ny, 64), plane stride Pp*nz
     float dl = 0.f, rho = 0.f;
     // launch shape of the fused kernel
-    int Q = 0, R = 0, zc = 0, gx = 0, gy = 0, threads = 0;
-    tlm_tuning tune = {1, 0, 0, 0};
-    int wq = 0, wr = 0, wseg = 0, wtasks = 0, wzc = 0, wnzc = 0, wblocks = 0; // wave kernel
+    tlm_tuning tune = {0, 0, 4, 1};
+    int strips = 0, zc = 0, nzc = 0, tasks = 0, blocks = 0;
     size_t epart_cap = 0;
     // state
     float *A = nullptr, *B = nullptr, *Vs = nullptr;
@@ -149,11 +148,11 @@ template <class T> int upload(tlm_ctx *c, T **dst, const T *src, size_t count)
     return TLM_OK;
 }
 
+// Device address of cell (k*ny + j)*nx + i: k*Pp + j*nx + i.
 long long padded_cell(const tlm_ctx *c, long long cell)
 {
-    long long i = cell % c->nx, jk = cell / c->nx;
-    long long j = jk % c->ny, k = jk / c->ny;
-    return (k * c->ny + j) * c->px + i;
+    long long k = cell / c->nxy;
+    return k * c->Pp + (cell - k * c->nxy);
 }
 
 bool cell_ok(const tlm_ctx *c, long long cell)
@@ -161,89 +160,45 @@ bool cell_ok(const tlm_ctx *c, long long cell)
     return cell >= 0 && cell < (long long)c->nx * c->ny * c->nz;
 }
 
-// Launch shapes.
-// Kernel 0 (workgroup rows): each workgroup holds R whole rows (Q quads
-// each) and marches over zc planes; zc is chosen so the grid has enough
-// workgroups to fill 256 CUs several times over.
-// Kernel 1 (independent waves): a wave holds rw row-strips of qw quads and
-// marches over wzc planes; by default about 16K wave tasks per launch.
+// Launch shape: a wave task is one 64-cell strip of a z-slab marched over
+// zc slabs; zc is chosen for about 64K tasks (8x the waves that fit on 256
+// CUs at 8 waves per SIMD), unless the tuning fixes it.
 void choose_shape(tlm_ctx *c)
 {
-    c->Q = c->px / 4;
-    c->R = c->tune.rows > 0 ? c->tune.rows : std::max(1, tlm::kMaxThreads / c->Q);
-    c->R = std::max(1, std::min(std::min(c->R, c->ny), tlm::kMaxThreads / c->Q));
-    c->threads = ((c->Q * c->R + 63) / 64) * 64;
-    c->gx = (c->ny + c->R - 1) / c->R;
-    const int target = 2048;
-    int nchunks = std::max(1, std::min(c->nz, (target + c->gx - 1) / c->gx));
-    c->zc = (c->nz + nchunks - 1) / nchunks;
-    if (c->tune.kernel == 0 && c->tune.planes > 0)
+    c->strips = (int)(c->Pp / tlm::kWave);
+    if (c->tune.planes > 0)
         c->zc = std::min(c->tune.planes, c->nz);
-    c->gy = (c->nz + c->zc - 1) / c->zc;
-
-    c->wq = std::min(c->Q, 64);
-    c->wr = 64 / c->wq;
-    c->wseg = (c->Q + c->wq - 1) / c->wq;
-    c->wtasks = ((c->ny + c->wr - 1) / c->wr) * c->wseg;
-    if (c->tune.kernel == 1 && c->tune.planes > 0)
-        c->wzc = std::min(c->tune.planes, c->nz);
     else
     {
-        const long long target_waves = 16384;
-        int nch = (int)std::max(1LL, std::min<long long>(c->nz, target_waves / c->wtasks));
-        c->wzc = (c->nz + nch - 1) / nch;
+        const long long target = 65536;
+        int nch = (int)std::max(1LL, std::min<long long>(c->nz, target / c->strips));
+        c->zc = (c->nz + nch - 1) / nch;
     }
-    c->wnzc = (c->nz + c->wzc - 1) / c->wzc;
-    c->wblocks = (int)(((long long)c->wtasks * c->wnzc + 3) / 4);
+    c->nzc = (c->nz + c->zc - 1) / c->zc;
+    c->tasks = c->strips * c->nzc;
+    const int wpb = std::max(1, std::min(4, c->tune.block_waves > 0 ? c->tune.block_waves : 4));
+    c->blocks = (c->tasks + wpb - 1) / wpb;
 }
 
 size_t energy_parts(const tlm_ctx *c)
 {
-    return c->tune.kernel == 1 ? (size_t)c->wtasks * c->wnzc : (size_t)c->gx * c->gy;
+    return (size_t)c->tasks;
 }
 
 template <bool S, bool E>
-void launch_wave(tlm_ctx *c, const tlm::WaveArgs &a)
+void launch_fused(tlm_ctx *c, const tlm::StepArgs &a)
 {
+    const int wpb = std::max(1, std::min(4, c->tune.block_waves > 0 ? c->tune.block_waves : 4));
+    dim3 grid(c->blocks), block(wpb * tlm::kWave);
     if (c->tune.nontemporal)
-        hipLaunchKernelGGL((tlm::k_step_wave<S, E, true>), dim3(c->wblocks), dim3(256), 0,
-                           c->stream, a);
+        hipLaunchKernelGGL((tlm::k_step<S, E, true>), grid, block, 0, c->stream, a);
     else
-        hipLaunchKernelGGL((tlm::k_step_wave<S, E, false>), dim3(c->wblocks), dim3(256), 0,
-                           c->stream, a);
+        hipLaunchKernelGGL((tlm::k_step<S, E, false>), grid, block, 0, c->stream, a);
 }
 
 int launch_step(tlm_ctx *c, int n)
 {
     const bool energy = c->every > 0 && (n % c->every) == 0;
-    if (c->tune.kernel == 1)
-    {
-        tlm::WaveArgs w;
-        w.A = c->A;
-        w.B = c->B;
-        w.Vs = c->Vs;
-        w.mat = c->mat;
-        w.props = c->props;
-        w.epart = c->epart;
-        w.nx = c->nx;
-        w.ny = c->ny;
-        w.nz = c->nz;
-        w.px = c->px;
-        w.Q = c->Q;
-        w.qw = c->wq;
-        w.rw = c->wr;
-        w.nseg = c->wseg;
-        w.tasks_xy = c->wtasks;
-        w.zc = c->wzc;
-        w.nzc = c->wnzc;
-        w.P = c->P;
-        w.rho = c->rho;
-        if (c->stubs)
-            energy ? launch_wave<true, true>(c, w) : launch_wave<true, false>(c, w);
-        else
-            energy ? launch_wave<false, true>(c, w) : launch_wave<false, false>(c, w);
-        return TLM_OK;
-    }
     tlm::StepArgs a;
     a.A = c->A;
     a.B = c->B;
@@ -254,27 +209,18 @@ int launch_step(tlm_ctx *c, int n)
     a.nx = c->nx;
     a.ny = c->ny;
     a.nz = c->nz;
-    a.px = c->px;
-    a.Q = c->Q;
-    a.R = c->R;
-    a.zc = c->zc;
+    a.nxy = (int)c->nxy;
+    a.Pp = c->Pp;
     a.P = c->P;
+    a.strips = c->strips;
+    a.zc = c->zc;
+    a.nzc = c->nzc;
+    a.tasks = c->tasks;
     a.rho = c->rho;
-    dim3 grid(c->gx, c->gy), block(c->threads);
     if (c->stubs)
-    {
-        if (energy)
-            hipLaunchKernelGGL((tlm::k_step<true, true>), grid, block, 0, c->stream, a);
-        else
-            hipLaunchKernelGGL((tlm::k_step<true, false>), grid, block, 0, c->stream, a);
-    }
+        energy ? launch_fused<true, true>(c, a) : launch_fused<true, false>(c, a);
     else
-    {
-        if (energy)
-            hipLaunchKernelGGL((tlm::k_step<false, true>), grid, block, 0, c->stream, a);
-        else
-            hipLaunchKernelGGL((tlm::k_step<false, false>), grid, block, 0, c->stream, a);
-    }
+        energy ? launch_fused<false, true>(c, a) : launch_fused<false, false>(c, a);
     return TLM_OK;
 }
 
@@ -408,12 +354,12 @@ int convert(tlm_ctx *c, const float *host_in, float *host_out, float *soa, int n
                 return hip_fail(c, e, "hipMemcpyAsync");
             }
             hipLaunchKernelGGL(tlm::k_aos_to_soa, dim3((unsigned)g), dim3(b), 0, c->stream, stage,
-                               soa, c->P, ncomp, c->nx, c->ny, c->px, c0, cnt);
+                               soa, c->P, ncomp, c->nxy, c->Pp, c0, cnt);
         }
         else
         {
             hipLaunchKernelGGL(tlm::k_soa_to_aos, dim3((unsigned)g), dim3(b), 0, c->stream, soa,
-                               stage, c->P, ncomp, c->nx, c->ny, c->px, c0, cnt);
+                               stage, c->P, ncomp, c->nxy, c->Pp, c0, cnt);
             hipError_t e = hipMemcpyAsync(host_out + c0 * ncomp, stage, cnt * ncomp * sizeof(float),
                                           hipMemcpyDeviceToHost, c->stream);
             if (e != hipSuccess)
@@ -466,8 +412,13 @@ int tlm_create(const tlm_grid_desc *d, tlm_ctx **out)
         return fail(nullptr, TLM_ERR_ARG, "empty grid");
     if (d->pulse_type != TLM_PULSE_F32)
         return fail(nullptr, TLM_ERR_UNSUPPORTED, "only FP32 pulse storage in this entry point");
-    if (((d->nx + 3) / 4) > tlm::kMaxThreads)
-        return fail(nullptr, TLM_ERR_UNSUPPORTED, "nx above 1024 cells is not supported");
+    {
+        // one pulse plane must be addressable with 32-bit byte offsets
+        const long long pp = ((long long)d->nx * d->ny + 63) / 64 * 64;
+        if (pp * (long long)d->nz * 4 > (1LL << 32) - (1LL << 20))
+            return fail(nullptr, TLM_ERR_UNSUPPORTED,
+                        "grid above ~1.07e9 cells: a pulse plane exceeds 32-bit byte offsets");
+    }
     if (!(d->dl > 0.0f))
         return fail(nullptr, TLM_ERR_ARG, "cell size must be positive");
     int ndev = tlm_device_count();
@@ -481,8 +432,9 @@ int tlm_create(const tlm_grid_desc *d, tlm_ctx **out)
     c->nx = d->nx;
     c->ny = d->ny;
     c->nz = d->nz;
-    c->px = (d->nx + 3) / 4 * 4;
-    c->P = (long long)c->px * c->ny * c->nz;
+    c->nxy = (long long)c->nx * c->ny;
+    c->Pp = (c->nxy + tlm::kWave - 1) / tlm::kWave * tlm::kWave;
+    c->P = c->Pp * c->nz;
     c->dl = d->dl;
     c->rho = d->rho;
     c->inv_dl = -1.0f / d->dl;
@@ -501,7 +453,7 @@ int tlm_create(const tlm_grid_desc *d, tlm_ctx **out)
         if ((e = hipEventCreate(ev)) != hipSuccess)
             return bail(hip_fail(c, e, "hipEventCreate"));
     int rc;
-    c->epart_cap = std::max((size_t)c->gx * c->gy, (size_t)c->wtasks * c->wnzc);
+    c->epart_cap = energy_parts(c);
     if ((rc = dalloc(c, &c->A, (size_t)c->P * 12)) || (rc = dalloc(c, &c->B, (size_t)c->P * 12)) ||
         (rc = dalloc(c, &c->mat, (size_t)c->P)) || (rc = dalloc(c, &c->epart, c->epart_cap)))
         return bail(rc);
@@ -551,12 +503,10 @@ int tlm_upload(tlm_ctx *c, const uint8_t *mat, const float *yhat, const float *g
         }
     if (max_id >= nmat)
         return fail(c, TLM_ERR_ARG, "material id without a yhat/ghat entry");
-    // padded material map: padding cells are PEC so they never link
+    // slab-padded material map; padding is marked PEC and is never a neighbour
     std::vector<uint8_t> pm((size_t)c->P, 1);
     for (long long k = 0; k < c->nz; ++k)
-        for (long long j = 0; j < c->ny; ++j)
-            std::memcpy(&pm[(size_t)((k * c->ny + j) * c->px)], mat + (k * c->ny + j) * c->nx,
-                        (size_t)c->nx);
+        std::memcpy(&pm[(size_t)(k * c->Pp)], mat + k * c->nxy, (size_t)c->nxy);
     HIPCHK(c, hipMemcpy(c->mat, pm.data(), pm.size(), hipMemcpyHostToDevice));
     // per-material (yh, gh, den), den = 4 + yh + gh as in TlmSolver.cpp:90
     std::vector<float4> props(std::max(1, nmat));
@@ -871,22 +821,22 @@ int tlm_launch_shape(const tlm_ctx *c, int *rows, int *planes, int *groups, int 
 {
     if (!c)
         return TLM_ERR_ARG;
-    const bool w = c->tune.kernel == 1;
     if (rows)
-        *rows = w ? c->wr : c->R;
+        *rows = c->strips;
     if (planes)
-        *planes = w ? c->wzc : c->zc;
+        *planes = c->zc;
     if (groups)
-        *groups = w ? c->wblocks : c->gx * c->gy;
+        *groups = c->blocks;
     if (threads)
-        *threads = w ? 256 : c->threads;
+        *threads = std::max(1, std::min(4, c->tune.block_waves > 0 ? c->tune.block_waves : 4)) *
+                   tlm::kWave;
     return TLM_OK;
 }
 
 int tlm_set_tuning(tlm_ctx *c, const tlm_tuning *t)
 {
-    if (!c || !t || t->kernel < 0 || t->kernel > 1 || t->planes < 0 || t->rows < 0 ||
-        t->nontemporal < 0 || t->nontemporal > 1)
+    if (!c || !t || t->kernel != 0 || t->planes < 0 || t->block_waves < 0 ||
+        t->block_waves > 4 || t->nontemporal < 0 || t->nontemporal > 1)
         return fail(c, TLM_ERR_ARG, "tlm_set_tuning: bad arguments");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->tune = *t;

@@ -1,13 +1,14 @@
 // tlm_kernels.h - device kernels of the TLM time step (FP32 pulses).
 //
-// Layout in HBM (see the context set-up in GpuTlmHip.hip):
-//   * pulses: 12 planes of P floats each (structure of arrays). Rows are
-//     padded to a pitch px = roundup(nx, 4) so every 4-cell quad is one
-//     16-byte aligned float4; padded cells carry material PEC and stay 0.
-//     Plane p holds port p of every cell, cell c = (k*ny + j)*px + i.
+// Layout in HBM (set up in GpuTlmHip.hip):
+//   * pulses: 12 planes of P floats each (structure of arrays). Inside a
+//     plane, z-slab k starts at k*Pp with Pp = roundup(nx*ny, 64), so every
+//     wavefront's 64 consecutive cells are one aligned 256-byte run; the
+//     cell (i, j, k) lives at k*Pp + j*nx + i. Port p of that cell is at
+//     p*P + k*Pp + j*nx + i. The slab padding is never read as a neighbour.
 //   * two such buffers (A = incident pulses of step n, B = of step n+1).
 //   * stubs: 3 planes of P floats, updated in place, dielectric cells only.
-//   * material: 1 byte per padded cell.
+//   * material: 1 byte per cell address.
 //
 // One step is (TlmSolver::step, TlmSolver.cpp:485-513):
 //   k_inject  : soft sources add 0.5*amp*sv(n) into A        (:487-505)
@@ -25,7 +26,8 @@
 namespace tlm {
 
 constexpr float ETA0 = 376.730313f; // TlmSolver.cpp:16
-constexpr int kMaxThreads = 256;
+constexpr int kWave = 64;
+constexpr int kNumXcd = 8;          // MI355X: 8 accelerator complex dies, one L2 each
 
 struct StepArgs
 {
@@ -34,685 +36,274 @@ struct StepArgs
     float *Vs;
     const uint8_t *mat;
     const float4 *props; // per dielectric id: (yh, gh, den, 0)
-    double *epart;       // energy partial per workgroup
+    double *epart;       // energy partial per wave task
     int nx, ny, nz;
-    int px;              // padded row pitch (cells)
-    int Q;               // quads per row = px/4
-    int R;               // rows per workgroup
-    int zc;              // planes per workgroup (z-march length)
-    long long P;         // floats per pulse plane = px*ny*nz
+    int nxy;             // nx*ny
+    long long Pp;        // slab pitch: roundup(nx*ny, 64)
+    long long P;         // floats per pulse plane = Pp*nz
+    int strips;          // 64-cell strips per slab: Pp/64
+    int zc, nzc;         // planes per task, number of z chunks
+    int tasks;           // strips * nzc
     float rho;
 };
 
-// Bits e (0..3) of a material quad that hold material `id`.
-__device__ __forceinline__ unsigned pec_bits(uint32_t m)
+// Loads and stores through a uniform plane base (SGPR pair) and a 32-bit
+// byte offset per lane (one VGPR), the global_load saddr+voffset form.
+__device__ __forceinline__ float ldo(const float *base, unsigned boff)
 {
-    unsigned b = 0;
-#pragma unroll
-    for (int e = 0; e < 4; ++e)
-        b |= (((m >> (8 * e)) & 0xFFu) == 1u ? 1u : 0u) << e;
-    return b;
+    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + boff);
 }
-__device__ __forceinline__ unsigned diel_bits(uint32_t m)
+template <bool NT> __device__ __forceinline__ void sto(float *base, unsigned boff, float v)
 {
-    unsigned b = 0;
-#pragma unroll
-    for (int e = 0; e < 4; ++e)
-        b |= (((m >> (8 * e)) & 0xFFu) >= 2u ? 1u : 0u) << e;
-    return b;
+    float *p = reinterpret_cast<float *>(reinterpret_cast<char *>(base) + boff);
+    if (NT)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
 }
-
-__device__ __forceinline__ void ld4(const float *p, float (&r)[4])
+__device__ __forceinline__ unsigned ldm(const uint8_t *mat, unsigned off)
 {
-    float4 x = *reinterpret_cast<const float4 *>(p);
-    r[0] = x.x;
-    r[1] = x.y;
-    r[2] = x.z;
-    r[3] = x.w;
-}
-__device__ __forceinline__ void st4(float *p, const float (&r)[4])
-{
-    *reinterpret_cast<float4 *>(p) = make_float4(r[0], r[1], r[2], r[3]);
-}
-// Store the elements of r whose bit is set in mask (a full float4 when all
-// four are set, the common case).
-__device__ __forceinline__ void st4_masked(float *p, const float (&r)[4], unsigned mask)
-{
-    if (mask == 0xFu)
-        st4(p, r);
-    else if (mask)
-    {
-#pragma unroll
-        for (int e = 0; e < 4; ++e)
-            if (mask & (1u << e))
-                p[e] = r[e];
-    }
+    return mat[off];
 }
 
-// Fused scatter + connect over a tile of R full rows x zc planes, marching
-// in z. Each thread owns one 4-cell quad of a row.
+// Workgroups are dispatched to the 8 XCDs round robin; renumber them so each
+// XCD owns one contiguous range of wave tasks. Neighbouring strips then share
+// an L2, which keeps the cache lines written from both sides of a strip seam
+// (and the neighbour-row material bytes) on one die.
+__device__ __forceinline__ int xcd_block(int b, int G)
+{
+    const int xcd = b % kNumXcd, local = b / kNumXcd;
+    const int per = G / kNumXcd, extra = G % kNumXcd;
+    return xcd * per + min(xcd, extra) + local;
+}
+
+// Fused scatter + connect. A wavefront owns a strip of 64 consecutive cells
+// of one z-slab (lane = cell) and marches over zc slabs.
 //
 // Every slot of B is written exactly once:
-//   x faces: by its owner; the neighbour's reflected pulse comes through LDS
-//            (rows are whole inside a workgroup).
-//   z faces: by its owner inside the z-march (the previous plane's z+
-//            pulses stay in registers, its z+ slots are stored one plane
-//            late); across z-chunk edges the sender pushes into the
-//            neighbour's slot.
+//   x faces: by its owner, with the neighbour's reflected pulse taken from
+//            the neighbouring lane; where a row continues into the next
+//            strip, the two seam cells push into each other's slot.
 //   y faces: the sender pushes its reflected pulse into the neighbour row's
-//            slot (aligned full-row float4 stores).
+//            slot.
+//   z faces: by the owner inside the z-march (the previous slab's z+
+//            pulses stay in registers and its z+ slots are stored one slab
+//            late); across z-chunk edges the sender pushes.
 // The owner writes its own slot instead whenever the link is broken: own
 // cell PEC (value kept, times rho on an outer face), outer face (rho * out),
 // neighbour PEC (-out). This is connect(), TlmSolver.cpp:563-583.
-template <bool STUBS, bool ENERGY>
-__global__ void __launch_bounds__(kMaxThreads) k_step(StepArgs a)
-{
-    __shared__ float4 xo[2][kMaxThreads]; // x- pulses of cell 0, x+ pulses of cell 3
-    __shared__ uint32_t xm[2][kMaxThreads];
-    __shared__ double ered[kMaxThreads / 64];
-
-    const int t = threadIdx.x;
-    const int r = t / a.Q;
-    const int q = t - r * a.Q;
-    const int j = blockIdx.x * a.R + r;
-    const bool act = (r < a.R) && (j < a.ny);
-    const int k0 = blockIdx.y * a.zc;
-    const int k1 = min(a.nz, k0 + a.zc);
-    const int i0 = 4 * q;
-    const long long sy = a.px;
-    const long long sz = (long long)a.px * a.ny;
-    const long long P = a.P;
-    const float rho = a.rho;
-
-    float zpo[2][4], zpv[2][4]; // previous plane: reflected z+ pulses and incident z+ pulses
-    uint32_t mprev = 0;
-    double esum = 0.0;
-
-    for (int k = k0; k < k1; ++k)
-    {
-        const int buf = (k - k0) & 1;
-        const long long c = ((long long)k * a.ny + j) * a.px + i0;
-        float v[12][4], o[12][4];
-        uint32_t m = 0x01010101u;
-        unsigned pm = 0xFu;
-
-        if (act)
-        {
-            m = *reinterpret_cast<const uint32_t *>(a.mat + c);
-            pm = pec_bits(m);
-#pragma unroll
-            for (int p = 0; p < 12; ++p)
-                ld4(a.A + p * P + c, v[p]);
-            if (ENERGY)
-            {
-#pragma unroll
-                for (int p = 0; p < 12; ++p)
-#pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        esum += (double)v[p][e] * (double)v[p][e];
-            }
-            unsigned dm = STUBS ? diel_bits(m) : 0u;
-            float vs[3][4];
-            if (STUBS && dm)
-            {
-#pragma unroll
-                for (int s = 0; s < 3; ++s)
-                    ld4(a.Vs + s * P + c, vs[s]);
-            }
-#pragma unroll
-            for (int e = 0; e < 4; ++e)
-            {
-                float vin[12], vout[12];
-#pragma unroll
-                for (int p = 0; p < 12; ++p)
-                    vin[p] = v[p][e];
-                if (STUBS && (dm & (1u << e)))
-                {
-                    float4 pr = a.props[((m >> (8 * e)) & 0xFFu) - 2u];
-                    float st[3] = {vs[0][e], vs[1][e], vs[2][e]};
-                    scn::scatter_diel(vin, st, pr.x, pr.z, vout);
-                    vs[0][e] = st[0];
-                    vs[1][e] = st[1];
-                    vs[2][e] = st[2];
-                }
-                else
-                    scn::scatter_air(vin, vout);
-#pragma unroll
-                for (int p = 0; p < 12; ++p)
-                    o[p][e] = vout[p];
-            }
-            if (STUBS && dm)
-            {
-#pragma unroll
-                for (int s = 0; s < 3; ++s)
-                    st4(a.Vs + s * P + c, vs[s]);
-            }
-            xo[buf][t] = make_float4(o[0][0], o[1][0], o[2][3], o[3][3]);
-            xm[buf][t] = m;
-        }
-        __syncthreads();
-        if (!act)
-            continue;
-
-        // ---- x faces (ports 0,1 = x-, 2,3 = x+): owner pull ----
-        {
-            float4 L = make_float4(0.f, 0.f, 0.f, 0.f), Rt = L;
-            unsigned pecL = 1u, pecR = 1u;
-            if (q > 0)
-            {
-                L = xo[buf][t - 1];
-                pecL = ((xm[buf][t - 1] >> 24) & 0xFFu) == 1u;
-            }
-            if (q < a.Q - 1)
-            {
-                Rt = xo[buf][t + 1];
-                pecR = (xm[buf][t + 1] & 0xFFu) == 1u;
-            }
-#pragma unroll
-            for (int pol = 0; pol < 2; ++pol)
-            {
-                float rm[4], rp[4];
-#pragma unroll
-                for (int e = 0; e < 4; ++e)
-                {
-                    const int i = i0 + e;
-                    const bool pec = (pm >> e) & 1u;
-                    // x- slot: neighbour i-1 sends its x+ pulse
-                    {
-                        const bool bnd = (i == 0);
-                        const float nbv = (e == 0) ? (pol ? L.w : L.z) : o[2 + pol][e - 1];
-                        const bool nbp = (e == 0) ? (pecL != 0u) : (((pm >> (e - 1)) & 1u) != 0u);
-                        const float oo = o[0 + pol][e], vv = v[0 + pol][e];
-                        rm[e] = pec ? (bnd ? rho * vv : vv) : (bnd ? rho * oo : (nbp ? -oo : nbv));
-                    }
-                    // x+ slot: neighbour i+1 sends its x- pulse
-                    {
-                        const bool bnd = (i == a.nx - 1);
-                        const float nbv = (e == 3) ? (pol ? Rt.y : Rt.x) : o[0 + pol][e + 1];
-                        const bool nbp = (e == 3) ? (pecR != 0u) : (((pm >> (e + 1)) & 1u) != 0u);
-                        const float oo = o[2 + pol][e], vv = v[2 + pol][e];
-                        rp[e] = pec ? (bnd ? rho * vv : vv) : (bnd ? rho * oo : (nbp ? -oo : nbv));
-                    }
-                }
-                st4(a.B + (0 + pol) * P + c, rm);
-                st4(a.B + (2 + pol) * P + c, rp);
-            }
-        }
-
-        // ---- y faces (ports 4,5 = y-, 6,7 = y+): sender push ----
-        {
-            const bool bLo = (j == 0), bHi = (j == a.ny - 1);
-            const unsigned pLo = bLo ? 0u : pec_bits(*reinterpret_cast<const uint32_t *>(a.mat + c - sy));
-            const unsigned pHi = bHi ? 0u : pec_bits(*reinterpret_cast<const uint32_t *>(a.mat + c + sy));
-            const unsigned ownLo = bLo ? 0xFu : (pm | pLo);
-            const unsigned pushLo = bLo ? 0u : (~pm & ~pLo & 0xFu);
-            const unsigned ownHi = bHi ? 0xFu : (pm | pHi);
-            const unsigned pushHi = bHi ? 0u : (~pm & ~pHi & 0xFu);
-#pragma unroll
-            for (int pol = 0; pol < 2; ++pol)
-            {
-                float rl[4], rh[4];
-#pragma unroll
-                for (int e = 0; e < 4; ++e)
-                {
-                    const bool pec = (pm >> e) & 1u;
-                    rl[e] = pec ? (bLo ? rho * v[4 + pol][e] : v[4 + pol][e])
-                                : (bLo ? rho * o[4 + pol][e] : -o[4 + pol][e]);
-                    rh[e] = pec ? (bHi ? rho * v[6 + pol][e] : v[6 + pol][e])
-                                : (bHi ? rho * o[6 + pol][e] : -o[6 + pol][e]);
-                }
-                st4_masked(a.B + (4 + pol) * P + c, rl, ownLo);
-                st4_masked(a.B + (6 + pol) * P + c - sy, o[4 + pol], pushLo);
-                st4_masked(a.B + (6 + pol) * P + c, rh, ownHi);
-                st4_masked(a.B + (4 + pol) * P + c + sy, o[6 + pol], pushHi);
-            }
-        }
-
-        // ---- z faces (ports 8,9 = z-, 10,11 = z+) ----
-        if (k > k0)
-        {
-            const unsigned pp = pec_bits(mprev);
-#pragma unroll
-            for (int pol = 0; pol < 2; ++pol)
-            {
-                float rz[4], rzp[4];
-#pragma unroll
-                for (int e = 0; e < 4; ++e)
-                {
-                    const bool pec = (pm >> e) & 1u, pecB = (pp >> e) & 1u;
-                    // this plane's z- slot: plane k-1 sends its z+ pulse
-                    rz[e] = pec ? v[8 + pol][e] : (pecB ? -o[8 + pol][e] : zpo[pol][e]);
-                    // plane k-1's z+ slot: this plane sends its z- pulse
-                    rzp[e] = pecB ? zpv[pol][e] : (pec ? -zpo[pol][e] : o[8 + pol][e]);
-                }
-                st4(a.B + (8 + pol) * P + c, rz);
-                st4(a.B + (10 + pol) * P + c - sz, rzp);
-            }
-        }
-        else if (k == 0)
-        {
-#pragma unroll
-            for (int pol = 0; pol < 2; ++pol)
-            {
-                float rz[4];
-#pragma unroll
-                for (int e = 0; e < 4; ++e)
-                    rz[e] = ((pm >> e) & 1u) ? rho * v[8 + pol][e] : rho * o[8 + pol][e];
-                st4(a.B + (8 + pol) * P + c, rz);
-            }
-        }
-        else
-        {
-            const unsigned pLo = pec_bits(*reinterpret_cast<const uint32_t *>(a.mat + c - sz));
-            const unsigned own = pm | pLo, push = ~pm & ~pLo & 0xFu;
-#pragma unroll
-            for (int pol = 0; pol < 2; ++pol)
-            {
-                float rz[4];
-#pragma unroll
-                for (int e = 0; e < 4; ++e)
-                    rz[e] = ((pm >> e) & 1u) ? v[8 + pol][e] : -o[8 + pol][e];
-                st4_masked(a.B + (8 + pol) * P + c, rz, own);
-                st4_masked(a.B + (10 + pol) * P + c - sz, o[8 + pol], push);
-            }
-        }
-        if (k == k1 - 1)
-        {
-            if (k == a.nz - 1)
-            {
-#pragma unroll
-                for (int pol = 0; pol < 2; ++pol)
-                {
-                    float rz[4];
-#pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        rz[e] = ((pm >> e) & 1u) ? rho * v[10 + pol][e] : rho * o[10 + pol][e];
-                    st4(a.B + (10 + pol) * P + c, rz);
-                }
-            }
-            else
-            {
-                const unsigned pHi = pec_bits(*reinterpret_cast<const uint32_t *>(a.mat + c + sz));
-                const unsigned own = pm | pHi, push = ~pm & ~pHi & 0xFu;
-#pragma unroll
-                for (int pol = 0; pol < 2; ++pol)
-                {
-                    float rz[4];
-#pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        rz[e] = ((pm >> e) & 1u) ? v[10 + pol][e] : -o[10 + pol][e];
-                    st4_masked(a.B + (10 + pol) * P + c, rz, own);
-                    st4_masked(a.B + (8 + pol) * P + c + sz, o[10 + pol], push);
-                }
-            }
-        }
-#pragma unroll
-        for (int pol = 0; pol < 2; ++pol)
-#pragma unroll
-            for (int e = 0; e < 4; ++e)
-            {
-                zpo[pol][e] = o[10 + pol][e];
-                zpv[pol][e] = v[10 + pol][e];
-            }
-        mprev = m;
-    }
-
-    if (ENERGY)
-    {
-        // fixed-order reduction: wave tree, then the waves in order
-        for (int off = 32; off > 0; off >>= 1)
-            esum += __shfl_down(esum, off, 64);
-        if ((t & 63) == 0)
-            ered[t >> 6] = esum;
-        __syncthreads();
-        if (t == 0)
-        {
-            double s = 0.0;
-            for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w)
-                s += ered[w];
-            a.epart[blockIdx.x + (size_t)blockIdx.y * gridDim.x] = s;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Wave-independent variant of the fused step: no LDS and no barrier.
-//
-// Each wavefront owns a strip of rows (qw quads per row-strip, rw strips per
-// wave) and marches over zc planes. x neighbours inside the wave are read
-// with lane shuffles; where a row is split between two waves (rows wider
-// than 256 cells) the two seam cells push their x pulse into each other's
-// slot, exactly like the y and z-chunk links. Incident pulses are not kept
-// live after the scatter: the rare PEC cells reload their (unchanged) value.
-// ---------------------------------------------------------------------------
-struct WaveArgs
-{
-    const float *A;
-    float *B;
-    float *Vs;
-    const uint8_t *mat;
-    const float4 *props;
-    double *epart; // energy partial per wave
-    int nx, ny, nz, px, Q;
-    int qw;        // quads per wave row-strip: min(Q, 64)
-    int rw;        // row-strips per wave: 64 / qw
-    int nseg;      // strips per row: ceil(Q / qw)
-    int tasks_xy;  // wave tasks per plane range: ceil(ny / rw) * nseg
-    int zc, nzc;   // planes per task, number of z chunks
-    long long P;
-    float rho;
-};
-
-typedef float f4v __attribute__((ext_vector_type(4)));
-
-template <bool NT> __device__ __forceinline__ void st4x(float *p, const float (&r)[4])
-{
-    if (NT)
-    {
-        f4v x = {r[0], r[1], r[2], r[3]};
-        __builtin_nontemporal_store(x, reinterpret_cast<f4v *>(p));
-    }
-    else
-        st4(p, r);
-}
-template <bool NT> __device__ __forceinline__ void st4x_masked(float *p, const float (&r)[4],
-                                                               unsigned mask)
-{
-    if (mask == 0xFu)
-        st4x<NT>(p, r);
-    else if (mask)
-    {
-#pragma unroll
-        for (int e = 0; e < 4; ++e)
-            if (mask & (1u << e))
-                p[e] = r[e];
-    }
-}
-__device__ __forceinline__ void ld4_if(const float *p, float (&r)[4], bool cond)
-{
-    if (cond)
-        ld4(p, r);
-    else
-        r[0] = r[1] = r[2] = r[3] = 0.0f;
-}
-__device__ __forceinline__ bool byte_is_pec(uint32_t m, int e)
-{
-    return ((m >> (8 * e)) & 0xFFu) == 1u;
-}
-
 template <bool STUBS, bool ENERGY, bool NT>
-__global__ void __launch_bounds__(kMaxThreads) k_step_wave(WaveArgs a)
+__global__ void __launch_bounds__(256) k_step(StepArgs a)
 {
-    const int lane = threadIdx.x & 63;
-    const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int tx = gw % a.tasks_xy, tz = gw / a.tasks_xy;
-    if (tz >= a.nzc)
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wpb = blockDim.x / kWave;
+    const int task = xcd_block(blockIdx.x, gridDim.x) * wpb + (threadIdx.x / kWave);
+    if (task >= a.tasks)
         return; // whole wave
-    const int r = lane / a.qw, ql = lane - r * a.qw;
-    const int jg = tx / a.nseg, s = tx - jg * a.nseg;
-    const int j = jg * a.rw + r, q = s * a.qw + ql;
-    const bool act = (r < a.rw) && (j < a.ny) && (q < a.Q);
-    const bool hasL = ql > 0, hasR = (ql < a.qw - 1) && (q < a.Q - 1);
-    const bool seamL = act && ql == 0 && q > 0;
-    const bool seamR = act && ql == a.qw - 1 && q < a.Q - 1;
+    const int tz = task / a.strips, t = task - tz * a.strips;
+    const int f = t * kWave + lane; // cell inside the slab
+    const bool act = f < a.nxy;
+    const int j = f / a.nx, i = f - j * a.nx;
     const int k0 = tz * a.zc, k1 = min(a.nz, k0 + a.zc);
-    const int i0 = 4 * q;
-    const long long sy = a.px, sz = (long long)a.px * a.ny, P = a.P;
     const float rho = a.rho;
-
-    float zpo[2][4]; // previous plane's reflected z+ pulses
-    uint32_t mprev = 0x01010101u;
-    double esum = 0.0;
-
-    for (int k = k0; k < k1; ++k)
+    // byte strides (all offsets of one plane fit in 32 bits)
+    const unsigned bz = (unsigned)a.Pp * 4u, by = (unsigned)a.nx * 4u;
+    const float *A[12];
+    float *B[12];
+#pragma unroll
+    for (int p = 0; p < 12; ++p)
     {
-        const long long c = ((long long)k * a.ny + j) * a.px + i0;
-        uint32_t m = 0x01010101u;
-        float o[12][4];
+        A[p] = a.A + p * a.P;
+        B[p] = a.B + p * a.P;
+    }
+    // x neighbour inside the wave, or across a strip seam
+    const bool hasL = act && lane > 0 && i > 0;
+    const bool seamL = act && lane == 0 && i > 0;
+    const bool hasR = act && lane < kWave - 1 && i < a.nx - 1;
+    const bool seamR = act && lane == kWave - 1 && i < a.nx - 1;
+    const bool bx0 = i == 0, bx1 = i == a.nx - 1, by0 = j == 0, by1 = j == a.ny - 1;
+
+    float zpo0 = 0.f, zpo1 = 0.f; // previous slab: reflected z+ pulses
+    bool pecPrev = false;
+    double esum = 0.0;
+    unsigned mo = (unsigned)(k0 * a.Pp + f); // cell index in the slab-padded space
+    unsigned bo = mo * 4u;                   // its byte offset inside a plane
+
+    for (int k = k0; k < k1; ++k, mo += (unsigned)a.Pp, bo += bz)
+    {
+        float o[12];
+        bool pec = true;
         if (act)
         {
-            m = *reinterpret_cast<const uint32_t *>(a.mat + c);
-            float v[12][4];
+            const unsigned m = ldm(a.mat, mo);
+            pec = (m == 1u);
+            float v[12];
 #pragma unroll
             for (int p = 0; p < 12; ++p)
-                ld4(a.A + p * P + c, v[p]);
+                v[p] = ldo(A[p], bo);
             if (ENERGY)
             {
 #pragma unroll
                 for (int p = 0; p < 12; ++p)
-#pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        esum += (double)v[p][e] * (double)v[p][e];
+                    esum += (double)v[p] * (double)v[p];
             }
-            const unsigned dm = STUBS ? diel_bits(m) : 0u;
-            float vs[3][4];
-            if (STUBS && dm)
+            if (STUBS && m >= 2u)
             {
-#pragma unroll
-                for (int t = 0; t < 3; ++t)
-                    ld4(a.Vs + t * P + c, vs[t]);
+                float *S0 = a.Vs, *S1 = a.Vs + a.P, *S2 = a.Vs + 2 * a.P;
+                float st[3] = {ldo(S0, bo), ldo(S1, bo), ldo(S2, bo)};
+                const float4 pr = a.props[m - 2u];
+                scn::scatter_diel(v, st, pr.x, pr.z, o);
+                sto<false>(S0, bo, st[0]);
+                sto<false>(S1, bo, st[1]);
+                sto<false>(S2, bo, st[2]);
             }
-#pragma unroll
-            for (int e = 0; e < 4; ++e)
-            {
-                float vin[12], vout[12];
-#pragma unroll
-                for (int p = 0; p < 12; ++p)
-                    vin[p] = v[p][e];
-                if (STUBS && (dm & (1u << e)))
-                {
-                    float4 pr = a.props[((m >> (8 * e)) & 0xFFu) - 2u];
-                    float st[3] = {vs[0][e], vs[1][e], vs[2][e]};
-                    scn::scatter_diel(vin, st, pr.x, pr.z, vout);
-                    vs[0][e] = st[0];
-                    vs[1][e] = st[1];
-                    vs[2][e] = st[2];
-                }
-                else
-                    scn::scatter_air(vin, vout);
-#pragma unroll
-                for (int p = 0; p < 12; ++p)
-                    o[p][e] = vout[p];
-            }
-            if (STUBS && dm)
-            {
-#pragma unroll
-                for (int t = 0; t < 3; ++t)
-                    st4(a.Vs + t * P + c, vs[t]);
-            }
+            else
+                scn::scatter_air(v, o);
         }
         else
         {
 #pragma unroll
             for (int p = 0; p < 12; ++p)
-#pragma unroll
-                for (int e = 0; e < 4; ++e)
-                    o[p][e] = 0.0f;
+                o[p] = 0.f;
         }
-        // x neighbours inside the wave (every lane takes part in the shuffles)
-        const float Lz = __shfl_up(o[2][3], 1, 64), Lw = __shfl_up(o[3][3], 1, 64);
-        const float Rx = __shfl_down(o[0][0], 1, 64), Ry = __shfl_down(o[1][0], 1, 64);
-        const uint32_t mL = __shfl_up(m, 1, 64), mR = __shfl_down(m, 1, 64);
-        if (!act)
-            continue;
-        const unsigned pm = pec_bits(m);
-
-        // ---- x faces (ports 0,1 = x-, 2,3 = x+) ----
+        // every lane takes part in the shuffles
+        const float L0 = __shfl_up(o[2], 1, kWave), L1 = __shfl_up(o[3], 1, kWave);
+        const float R0 = __shfl_down(o[0], 1, kWave), R1 = __shfl_down(o[1], 1, kWave);
+        const int pecL_lane = __shfl_up((int)pec, 1, kWave);
+        const int pecR_lane = __shfl_down((int)pec, 1, kWave);
+        if (act)
         {
-            bool pecL = true, pecR = true;
-            if (hasL)
-                pecL = byte_is_pec(mL, 3);
-            else if (seamL)
-                pecL = a.mat[c - 1] == 1;
-            if (hasR)
-                pecR = byte_is_pec(mR, 0);
-            else if (seamR)
-                pecR = a.mat[c + 4] == 1;
-            const bool pec0 = pm & 1u, pec3 = (pm >> 3) & 1u;
-            const unsigned maskM = seamL ? (0xEu | ((pec0 || pecL) ? 1u : 0u)) : 0xFu;
-            const unsigned maskP = seamR ? (0x7u | ((pec3 || pecR) ? 8u : 0u)) : 0xFu;
-#pragma unroll
-            for (int pol = 0; pol < 2; ++pol)
+            // ---- x faces: ports 0,1 (x-) and 2,3 (x+) ----
             {
-                float vm[4], vp[4], rm[4], rp[4];
-                ld4_if(a.A + (0 + pol) * P + c, vm, pm != 0u);
-                ld4_if(a.A + (2 + pol) * P + c, vp, pm != 0u);
-#pragma unroll
-                for (int e = 0; e < 4; ++e)
-                {
-                    const int i = i0 + e;
-                    const bool pec = (pm >> e) & 1u;
-                    {
-                        const bool bnd = (i == 0);
-                        const float nbv = (e == 0) ? (pol ? Lw : Lz) : o[2 + pol][e - 1];
-                        const bool nbp = (e == 0) ? pecL : (((pm >> (e - 1)) & 1u) != 0u);
-                        const float oo = o[0 + pol][e];
-                        rm[e] = pec ? (bnd ? rho * vm[e] : vm[e])
-                                    : (bnd ? rho * oo : (nbp ? -oo : nbv));
-                    }
-                    {
-                        const bool bnd = (i == a.nx - 1);
-                        const float nbv = (e == 3) ? (pol ? Ry : Rx) : o[0 + pol][e + 1];
-                        const bool nbp = (e == 3) ? pecR : (((pm >> (e + 1)) & 1u) != 0u);
-                        const float oo = o[2 + pol][e];
-                        rp[e] = pec ? (bnd ? rho * vp[e] : vp[e])
-                                    : (bnd ? rho * oo : (nbp ? -oo : nbv));
-                    }
-                }
-                st4x_masked<NT>(a.B + (0 + pol) * P + c, rm, maskM);
-                st4x_masked<NT>(a.B + (2 + pol) * P + c, rp, maskP);
-                if (seamL && !pec0 && !pecL)
-                    a.B[(2 + pol) * P + c - 1] = o[0 + pol][0];
-                if (seamR && !pec3 && !pecR)
-                    a.B[(0 + pol) * P + c + 4] = o[2 + pol][3];
-            }
-        }
-
-        // ---- y faces (ports 4,5 = y-, 6,7 = y+): sender push ----
-        {
-            const bool bLo = (j == 0), bHi = (j == a.ny - 1);
-            const unsigned pLo = bLo ? 0u : pec_bits(*reinterpret_cast<const uint32_t *>(a.mat + c - sy));
-            const unsigned pHi = bHi ? 0u : pec_bits(*reinterpret_cast<const uint32_t *>(a.mat + c + sy));
-            const unsigned ownLo = bLo ? 0xFu : (pm | pLo);
-            const unsigned pushLo = bLo ? 0u : (~pm & ~pLo & 0xFu);
-            const unsigned ownHi = bHi ? 0xFu : (pm | pHi);
-            const unsigned pushHi = bHi ? 0u : (~pm & ~pHi & 0xFu);
-#pragma unroll
-            for (int pol = 0; pol < 2; ++pol)
-            {
-                float vl[4], vh[4], rl[4], rh[4];
-                ld4_if(a.A + (4 + pol) * P + c, vl, pm != 0u);
-                ld4_if(a.A + (6 + pol) * P + c, vh, pm != 0u);
-#pragma unroll
-                for (int e = 0; e < 4; ++e)
-                {
-                    const bool pec = (pm >> e) & 1u;
-                    rl[e] = pec ? (bLo ? rho * vl[e] : vl[e]) : (bLo ? rho * o[4 + pol][e] : -o[4 + pol][e]);
-                    rh[e] = pec ? (bHi ? rho * vh[e] : vh[e]) : (bHi ? rho * o[6 + pol][e] : -o[6 + pol][e]);
-                }
-                st4x_masked<NT>(a.B + (4 + pol) * P + c, rl, ownLo);
-                st4x_masked<NT>(a.B + (6 + pol) * P + c - sy, o[4 + pol], pushLo);
-                st4x_masked<NT>(a.B + (6 + pol) * P + c, rh, ownHi);
-                st4x_masked<NT>(a.B + (4 + pol) * P + c + sy, o[6 + pol], pushHi);
-            }
-        }
-
-        // ---- z faces (ports 8,9 = z-, 10,11 = z+) ----
-        if (k > k0)
-        {
-            const unsigned pp = pec_bits(mprev);
-#pragma unroll
-            for (int pol = 0; pol < 2; ++pol)
-            {
-                float vz[4], vzp[4], rz[4], rzp[4];
-                ld4_if(a.A + (8 + pol) * P + c, vz, pm != 0u);
-                ld4_if(a.A + (10 + pol) * P + c - sz, vzp, pp != 0u);
-#pragma unroll
-                for (int e = 0; e < 4; ++e)
-                {
-                    const bool pec = (pm >> e) & 1u, pecB = (pp >> e) & 1u;
-                    rz[e] = pec ? vz[e] : (pecB ? -o[8 + pol][e] : zpo[pol][e]);
-                    rzp[e] = pecB ? vzp[e] : (pec ? -zpo[pol][e] : o[8 + pol][e]);
-                }
-                st4x<NT>(a.B + (8 + pol) * P + c, rz);
-                st4x<NT>(a.B + (10 + pol) * P + c - sz, rzp);
-            }
-        }
-        else if (k == 0)
-        {
-#pragma unroll
-            for (int pol = 0; pol < 2; ++pol)
-            {
-                float vz[4], rz[4];
-                ld4_if(a.A + (8 + pol) * P + c, vz, pm != 0u);
-#pragma unroll
-                for (int e = 0; e < 4; ++e)
-                    rz[e] = ((pm >> e) & 1u) ? rho * vz[e] : rho * o[8 + pol][e];
-                st4x<NT>(a.B + (8 + pol) * P + c, rz);
-            }
-        }
-        else
-        {
-            const unsigned pLo = pec_bits(*reinterpret_cast<const uint32_t *>(a.mat + c - sz));
-            const unsigned own = pm | pLo, push = ~pm & ~pLo & 0xFu;
-#pragma unroll
-            for (int pol = 0; pol < 2; ++pol)
-            {
-                float vz[4], rz[4];
-                ld4_if(a.A + (8 + pol) * P + c, vz, pm != 0u);
-#pragma unroll
-                for (int e = 0; e < 4; ++e)
-                    rz[e] = ((pm >> e) & 1u) ? vz[e] : -o[8 + pol][e];
-                st4x_masked<NT>(a.B + (8 + pol) * P + c, rz, own);
-                st4x_masked<NT>(a.B + (10 + pol) * P + c - sz, o[8 + pol], push);
-            }
-        }
-        if (k == k1 - 1)
-        {
-            if (k == a.nz - 1)
-            {
+                const bool pecL = hasL ? (pecL_lane != 0) : (seamL ? ldm(a.mat, mo - 1) == 1u : true);
+                const bool pecR = hasR ? (pecR_lane != 0) : (seamR ? ldm(a.mat, mo + 1) == 1u : true);
 #pragma unroll
                 for (int pol = 0; pol < 2; ++pol)
                 {
-                    float vz[4], rz[4];
-                    ld4_if(a.A + (10 + pol) * P + c, vz, pm != 0u);
-#pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        rz[e] = ((pm >> e) & 1u) ? rho * vz[e] : rho * o[10 + pol][e];
-                    st4x<NT>(a.B + (10 + pol) * P + c, rz);
+                    const float oo = o[0 + pol];
+                    if (pec)
+                    {
+                        const float old = ldo(A[0 + pol], bo);
+                        sto<NT>(B[0 + pol], bo, bx0 ? rho * old : old);
+                    }
+                    else if (bx0)
+                        sto<NT>(B[0 + pol], bo, rho * oo);
+                    else if (pecL)
+                        sto<NT>(B[0 + pol], bo, -oo);
+                    else if (hasL)
+                        sto<NT>(B[0 + pol], bo, pol ? L1 : L0);
+                    else // seam with a live neighbour: it fills my slot, I fill its
+                        sto<NT>(B[2 + pol], bo - 4u, oo);
+
+                    const float op = o[2 + pol];
+                    if (pec)
+                    {
+                        const float old = ldo(A[2 + pol], bo);
+                        sto<NT>(B[2 + pol], bo, bx1 ? rho * old : old);
+                    }
+                    else if (bx1)
+                        sto<NT>(B[2 + pol], bo, rho * op);
+                    else if (pecR)
+                        sto<NT>(B[2 + pol], bo, -op);
+                    else if (hasR)
+                        sto<NT>(B[2 + pol], bo, pol ? R1 : R0);
+                    else
+                        sto<NT>(B[0 + pol], bo + 4u, op);
                 }
             }
-            else
+            // ---- y faces: ports 4,5 (y-) and 6,7 (y+), sender push ----
             {
-                const unsigned pHi = pec_bits(*reinterpret_cast<const uint32_t *>(a.mat + c + sz));
-                const unsigned own = pm | pHi, push = ~pm & ~pHi & 0xFu;
+                const bool pecLo = by0 ? false : ldm(a.mat, mo - (unsigned)a.nx) == 1u;
+                const bool pecHi = by1 ? false : ldm(a.mat, mo + (unsigned)a.nx) == 1u;
 #pragma unroll
                 for (int pol = 0; pol < 2; ++pol)
                 {
-                    float vz[4], rz[4];
-                    ld4_if(a.A + (10 + pol) * P + c, vz, pm != 0u);
+                    const float ol = o[4 + pol], oh = o[6 + pol];
+                    if (pec)
+                    {
+                        const float vl = ldo(A[4 + pol], bo), vh = ldo(A[6 + pol], bo);
+                        sto<NT>(B[4 + pol], bo, by0 ? rho * vl : vl);
+                        sto<NT>(B[6 + pol], bo, by1 ? rho * vh : vh);
+                    }
+                    else
+                    {
+                        if (by0)
+                            sto<NT>(B[4 + pol], bo, rho * ol);
+                        else if (pecLo)
+                            sto<NT>(B[4 + pol], bo, -ol);
+                        else
+                            sto<NT>(B[6 + pol], bo - by, ol);
+                        if (by1)
+                            sto<NT>(B[6 + pol], bo, rho * oh);
+                        else if (pecHi)
+                            sto<NT>(B[6 + pol], bo, -oh);
+                        else
+                            sto<NT>(B[4 + pol], bo + by, oh);
+                    }
+                }
+            }
+            // ---- z faces: ports 8,9 (z-) and 10,11 (z+) ----
 #pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        rz[e] = ((pm >> e) & 1u) ? vz[e] : -o[10 + pol][e];
-                    st4x_masked<NT>(a.B + (10 + pol) * P + c, rz, own);
-                    st4x_masked<NT>(a.B + (8 + pol) * P + c + sz, o[10 + pol], push);
+            for (int pol = 0; pol < 2; ++pol)
+            {
+                const float ol = o[8 + pol];
+                if (k > k0)
+                {
+                    // pull: slab k-1 is in registers
+                    const float zp = pol ? zpo1 : zpo0;
+                    if (pec)
+                        sto<NT>(B[8 + pol], bo, ldo(A[8 + pol], bo));
+                    else
+                        sto<NT>(B[8 + pol], bo, pecPrev ? -ol : zp);
+                    // slab k-1's z+ slot
+                    if (pecPrev)
+                        sto<NT>(B[10 + pol], bo - bz, ldo(A[10 + pol], bo - bz));
+                    else
+                        sto<NT>(B[10 + pol], bo - bz, pec ? -zp : ol);
+                }
+                else if (k == 0)
+                {
+                    sto<NT>(B[8 + pol], bo, rho * (pec ? ldo(A[8 + pol], bo) : ol));
+                }
+                else
+                {
+                    const bool pecLo = ldm(a.mat, mo - (unsigned)a.Pp) == 1u;
+                    if (pec)
+                        sto<NT>(B[8 + pol], bo, ldo(A[8 + pol], bo));
+                    else if (pecLo)
+                        sto<NT>(B[8 + pol], bo, -ol);
+                    else
+                        sto<NT>(B[10 + pol], bo - bz, ol);
+                }
+                if (k == k1 - 1)
+                {
+                    const float oh = o[10 + pol];
+                    if (k == a.nz - 1)
+                        sto<NT>(B[10 + pol], bo, rho * (pec ? ldo(A[10 + pol], bo) : oh));
+                    else
+                    {
+                        const bool pecHi = ldm(a.mat, mo + (unsigned)a.Pp) == 1u;
+                        if (pec)
+                            sto<NT>(B[10 + pol], bo, ldo(A[10 + pol], bo));
+                        else if (pecHi)
+                            sto<NT>(B[10 + pol], bo, -oh);
+                        else
+                            sto<NT>(B[8 + pol], bo + bz, oh);
+                    }
                 }
             }
         }
-#pragma unroll
-        for (int pol = 0; pol < 2; ++pol)
-#pragma unroll
-            for (int e = 0; e < 4; ++e)
-                zpo[pol][e] = o[10 + pol][e];
-        mprev = m;
+        zpo0 = o[10];
+        zpo1 = o[11];
+        pecPrev = pec;
     }
 
     if (ENERGY)
     {
-        for (int off = 32; off > 0; off >>= 1)
-            esum += __shfl_down(esum, off, 64);
+        for (int off = kWave / 2; off > 0; off >>= 1)
+            esum += __shfl_down(esum, off, kWave);
         if (lane == 0)
-            a.epart[gw] = esum;
+            a.epart[task] = esum;
     }
 }
 
@@ -910,30 +501,28 @@ __global__ void k_probe_dft(const float *rec, int npr, const double *ctab, const
 
 // AoS (12 or 3 floats per cell, unpadded) <-> SoA padded planes, for the
 // cells [c0, c0 + count) of the unpadded index space.
-__global__ void k_aos_to_soa(const float *aos, float *soa, long long P, int ncomp, int nx, int ny,
-                             int px, long long c0, long long count)
+__global__ void k_aos_to_soa(const float *aos, float *soa, long long P, int ncomp, long long nxy,
+                             long long Pp, long long c0, long long count)
 {
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= count)
         return;
     const long long cell = c0 + id;
-    const long long i = cell % nx, jk = cell / nx;
-    const long long j = jk % ny, k = jk / ny;
-    const long long pc = (k * ny + j) * px + i;
+    const long long k = cell / nxy;
+    const long long pc = k * Pp + (cell - k * nxy);
     for (int p = 0; p < ncomp; ++p)
         soa[p * P + pc] = aos[id * ncomp + p];
 }
 
-__global__ void k_soa_to_aos(const float *soa, float *aos, long long P, int ncomp, int nx, int ny,
-                             int px, long long c0, long long count)
+__global__ void k_soa_to_aos(const float *soa, float *aos, long long P, int ncomp, long long nxy,
+                             long long Pp, long long c0, long long count)
 {
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= count)
         return;
     const long long cell = c0 + id;
-    const long long i = cell % nx, jk = cell / nx;
-    const long long j = jk % ny, k = jk / ny;
-    const long long pc = (k * ny + j) * px + i;
+    const long long k = cell / nxy;
+    const long long pc = k * Pp + (cell - k * nxy);
     for (int p = 0; p < ncomp; ++p)
         aos[id * ncomp + p] = soa[p * P + pc];
 }

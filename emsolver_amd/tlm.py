@@ -235,21 +235,21 @@ class TlmDevice:
                                           C.byref(ker) if kernel else None), "tlm_time_steps")
         return tot.value, (ker.value if kernel else None)
 
-    def set_tuning(self, kernel=1, planes=0, rows=0, nontemporal=0):
-        t = _native.Tuning(int(kernel), int(planes), int(rows), int(nontemporal))
+    def set_tuning(self, kernel=0, planes=0, block_waves=0, nontemporal=1):
+        t = _native.Tuning(int(kernel), int(planes), int(block_waves), int(nontemporal))
         self._chk(self.lib.tlm_set_tuning(self.h, C.byref(t)), "tlm_set_tuning")
 
     def get_tuning(self):
         t = _native.Tuning()
         self._chk(self.lib.tlm_get_tuning(self.h, C.byref(t)), "tlm_get_tuning")
-        return {"kernel": t.kernel, "planes": t.planes, "rows": t.rows,
+        return {"kernel": t.kernel, "planes": t.planes, "block_waves": t.block_waves,
                 "nontemporal": t.nontemporal}
 
     def launch_shape(self):
         r, z, g, t = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         self._chk(self.lib.tlm_launch_shape(self.h, C.byref(r), C.byref(z), C.byref(g),
                                             C.byref(t)), "tlm_launch_shape")
-        return {"rows_per_group": r.value, "planes_per_group": z.value, "groups": g.value,
+        return {"strips_per_slab": r.value, "slabs_per_task": z.value, "groups": g.value,
                 "threads_per_group": t.value}
 
     # ---- readback ----

@@ -165,18 +165,17 @@ int tlm_copy_bandwidth(int device, size_t bytes, int iters, float *gbps, float *
 /* Tuning of the fused scatter/connect kernel (no effect on results, which
  * are bit-identical for every setting). No counterpart in the reference. */
 typedef struct tlm_tuning {
-    int32_t kernel;      /* 0: workgroups of whole rows, x exchange through LDS;
-                            1: independent wavefronts, x exchange by lane shuffles (default) */
-    int32_t planes;      /* z planes marched per task; 0 = automatic */
-    int32_t rows;        /* rows per workgroup for kernel 0; 0 = automatic */
-    int32_t nontemporal; /* 1: non-temporal stores of the next-step pulses */
+    int32_t kernel;      /* 0: one cell per lane, 64-cell strips per wavefront (only variant) */
+    int32_t planes;      /* z slabs marched per wave task; 0 = automatic */
+    int32_t block_waves; /* wavefronts per workgroup, 1..4; 0 = 4 */
+    int32_t nontemporal; /* 1 (default): non-temporal stores of the next-step pulses */
 } tlm_tuning;
 
 int tlm_set_tuning(tlm_ctx *ctx, const tlm_tuning *t);
 int tlm_get_tuning(const tlm_ctx *ctx, tlm_tuning *t);
 
-/* Launch shape chosen for the fused kernel: rows per workgroup, planes per
- * workgroup, workgroup count. For reporting. */
+/* Launch shape chosen for the fused kernel: strips per slab, slabs per wave
+ * task, workgroup count, threads per workgroup. For reporting. */
 int tlm_launch_shape(const tlm_ctx *ctx, int *rows_per_group, int *planes_per_group,
                      int *groups, int *threads_per_group);
 

@@ -51,8 +51,8 @@ def test_argument_checks_before_device():
     assert _create(4, 4, 4, dl=0.0)[0] == -1
     rc, _, msg = _create(4, 4, 4, pulse=7)
     assert rc == -6 and "FP32" in msg
-    rc, _, msg = _create(2000, 4, 4)
-    assert rc == -6 and "1024" in msg
+    rc, _, msg = _create(1100, 1100, 1000)
+    assert rc == -6 and "32-bit" in msg
     lib = _native.load()
     assert lib.tlm_create(None, None) == -1
 

@@ -115,11 +115,10 @@ def test_free_space_64_full_run_tolerances():
 
 
 TUNINGS = [
-    {"kernel": 0, "planes": 0, "rows": 0, "nontemporal": 0},
-    {"kernel": 0, "planes": 3, "rows": 1, "nontemporal": 0},
-    {"kernel": 1, "planes": 0, "rows": 0, "nontemporal": 0},
-    {"kernel": 1, "planes": 1, "rows": 0, "nontemporal": 1},
-    {"kernel": 1, "planes": 5, "rows": 0, "nontemporal": 0},
+    {"kernel": 0, "planes": 0, "block_waves": 0, "nontemporal": 1},
+    {"kernel": 0, "planes": 1, "block_waves": 1, "nontemporal": 0},
+    {"kernel": 0, "planes": 3, "block_waves": 2, "nontemporal": 1},
+    {"kernel": 0, "planes": 64, "block_waves": 4, "nontemporal": 0},
 ]
 
 
@@ -132,11 +131,12 @@ def _run_tuned(sc, tuning, steps=None):
         return tlm.collect(dev, sc, steps)
 
 
-@pytest.mark.parametrize("tuning", TUNINGS, ids=lambda t: "k{kernel}_z{planes}_r{rows}_nt{nontemporal}".format(**t))
-@pytest.mark.parametrize("shape", [(12, 9, 11), (300, 5, 7), (600, 3, 4), (1024, 2, 3)])
+@pytest.mark.parametrize("tuning", TUNINGS, ids=lambda t: "z{planes}_w{block_waves}_nt{nontemporal}".format(**t))
+@pytest.mark.parametrize("shape", [(12, 9, 11), (300, 5, 7), (63, 7, 5), (2000, 2, 3)])
 def test_tunings_bitexact(shape, tuning):
-    """Every launch shape / kernel variant gives the oracle's bits, including
-    rows split across 2-4 waves (nx = 300, 600, 1024) where seam cells push."""
+    """Every launch shape gives the oracle's bits, including rows that start
+    mid-strip (nx = 12, 63, 300) and rows split across many strips
+    (nx = 300, 2000) where the seam cells push."""
     sc = _random_scene(*shape, seed=shape[0] + 7, rho=0.37, steps=10)
     print(f"[parity] {sc.name} tuning {tuning}", flush=True)
     g = _run_tuned(sc, tuning)
