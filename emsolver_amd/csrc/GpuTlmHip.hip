@@ -39,12 +39,12 @@ struct tlm_ctx
     long long nxy = 0, Pp = 0, P = 0; // slab pitch roundup(nx*ny, 64), plane stride Pp*nz
     float dl = 0.f, rho = 0.f;
     // launch shape of the fused kernel
-    tlm_tuning tune = {0, 0, 4, 1};
-    int strips = 0, zc = 0, nzc = 0, tasks = 0, blocks = 0;
+    tlm_tuning tune = {0, 1, 4, 0, 0};
+    int strips = 0, zc = 0, nzc = 0, tasks = 0, groups = 0, blocks = 0, wpb = 4;
     size_t epart_cap = 0;
     // state
     float *A = nullptr, *B = nullptr, *Vs = nullptr;
-    uint8_t *mat = nullptr;
+    uint16_t *code = nullptr; // cell code: material + neighbour-PEC bits
     float4 *props = nullptr;
     int nmat = 0;
     bool stubs = false, uploaded = false;
@@ -161,8 +161,9 @@ bool cell_ok(const tlm_ctx *c, long long cell)
 }
 
 // Launch shape: a wave task is one 64-cell strip of a z-slab marched over
-// zc slabs; zc is chosen for about 64K tasks (8x the waves that fit on 256
-// CUs at 8 waves per SIMD), unless the tuning fixes it.
+// zc slabs (tuning.planes; 0 picks zc for about 64K tasks). Task groups of
+// block_waves tasks go to workgroups; tuning.grid_blocks > 0 caps the grid
+// (persistent workgroups walking their XCD's range of groups).
 void choose_shape(tlm_ctx *c)
 {
     c->strips = (int)(c->Pp / tlm::kWave);
@@ -176,20 +177,20 @@ void choose_shape(tlm_ctx *c)
     }
     c->nzc = (c->nz + c->zc - 1) / c->zc;
     c->tasks = c->strips * c->nzc;
-    const int wpb = std::max(1, std::min(4, c->tune.block_waves > 0 ? c->tune.block_waves : 4));
-    c->blocks = (c->tasks + wpb - 1) / wpb;
+    c->wpb = std::max(1, std::min(4, c->tune.block_waves > 0 ? c->tune.block_waves : 4));
+    c->groups = (c->tasks + c->wpb - 1) / c->wpb;
+    c->blocks = c->tune.grid_blocks > 0 ? std::min(c->groups, c->tune.grid_blocks) : c->groups;
 }
 
 size_t energy_parts(const tlm_ctx *c)
 {
-    return (size_t)c->tasks;
+    return (size_t)c->blocks * c->wpb;
 }
 
 template <bool S, bool E>
 void launch_fused(tlm_ctx *c, const tlm::StepArgs &a)
 {
-    const int wpb = std::max(1, std::min(4, c->tune.block_waves > 0 ? c->tune.block_waves : 4));
-    dim3 grid(c->blocks), block(wpb * tlm::kWave);
+    dim3 grid(c->blocks), block(c->wpb * tlm::kWave);
     if (c->tune.nontemporal)
         hipLaunchKernelGGL((tlm::k_step<S, E, true>), grid, block, 0, c->stream, a);
     else
@@ -203,7 +204,7 @@ int launch_step(tlm_ctx *c, int n)
     a.A = c->A;
     a.B = c->B;
     a.Vs = c->Vs;
-    a.mat = c->mat;
+    a.code = c->code;
     a.props = c->props;
     a.epart = c->epart;
     a.nx = c->nx;
@@ -216,6 +217,7 @@ int launch_step(tlm_ctx *c, int n)
     a.zc = c->zc;
     a.nzc = c->nzc;
     a.tasks = c->tasks;
+    a.groups = c->groups;
     a.rho = c->rho;
     if (c->stubs)
         energy ? launch_fused<true, true>(c, a) : launch_fused<true, false>(c, a);
@@ -455,7 +457,7 @@ int tlm_create(const tlm_grid_desc *d, tlm_ctx **out)
     int rc;
     c->epart_cap = energy_parts(c);
     if ((rc = dalloc(c, &c->A, (size_t)c->P * 12)) || (rc = dalloc(c, &c->B, (size_t)c->P * 12)) ||
-        (rc = dalloc(c, &c->mat, (size_t)c->P)) || (rc = dalloc(c, &c->epart, c->epart_cap)))
+        (rc = dalloc(c, &c->code, (size_t)c->P)) || (rc = dalloc(c, &c->epart, c->epart_cap)))
         return bail(rc);
     *out = c;
     return TLM_OK;
@@ -503,11 +505,37 @@ int tlm_upload(tlm_ctx *c, const uint8_t *mat, const float *yhat, const float *g
         }
     if (max_id >= nmat)
         return fail(c, TLM_ERR_ARG, "material id without a yhat/ghat entry");
-    // slab-padded material map; padding is marked PEC and is never a neighbour
-    std::vector<uint8_t> pm((size_t)c->P, 1);
-    for (long long k = 0; k < c->nz; ++k)
-        std::memcpy(&pm[(size_t)(k * c->Pp)], mat + k * c->nxy, (size_t)c->nxy);
-    HIPCHK(c, hipMemcpy(c->mat, pm.data(), pm.size(), hipMemcpyHostToDevice));
+    // slab-padded cell codes: material id | neighbour-PEC bits; padding is
+    // marked PEC and is never anybody's neighbour
+    std::vector<uint16_t> code((size_t)c->P, 1);
+    {
+        const int nx = c->nx, ny = c->ny, nz = c->nz;
+        const long long sy = nx, sz = c->nxy;
+        for (long long k = 0; k < nz; ++k)
+            for (long long j = 0; j < ny; ++j)
+            {
+                const uint8_t *row = mat + k * sz + j * sy;
+                uint16_t *out = &code[(size_t)(k * c->Pp + j * sy)];
+                for (long long i = 0; i < nx; ++i)
+                {
+                    unsigned v = row[i];
+                    if (i > 0 && row[i - 1] == 1)
+                        v |= tlm::NB_XM;
+                    if (i < nx - 1 && row[i + 1] == 1)
+                        v |= tlm::NB_XP;
+                    if (j > 0 && row[i - sy] == 1)
+                        v |= tlm::NB_YM;
+                    if (j < ny - 1 && row[i + sy] == 1)
+                        v |= tlm::NB_YP;
+                    if (k > 0 && row[i - sz] == 1)
+                        v |= tlm::NB_ZM;
+                    if (k < nz - 1 && row[i + sz] == 1)
+                        v |= tlm::NB_ZP;
+                    out[i] = (uint16_t)v;
+                }
+            }
+    }
+    HIPCHK(c, hipMemcpy(c->code, code.data(), code.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
     // per-material (yh, gh, den), den = 4 + yh + gh as in TlmSolver.cpp:90
     std::vector<float4> props(std::max(1, nmat));
     for (int i = 0; i < nmat; ++i)
@@ -828,15 +856,14 @@ int tlm_launch_shape(const tlm_ctx *c, int *rows, int *planes, int *groups, int 
     if (groups)
         *groups = c->blocks;
     if (threads)
-        *threads = std::max(1, std::min(4, c->tune.block_waves > 0 ? c->tune.block_waves : 4)) *
-                   tlm::kWave;
+        *threads = c->wpb * tlm::kWave;
     return TLM_OK;
 }
 
 int tlm_set_tuning(tlm_ctx *c, const tlm_tuning *t)
 {
     if (!c || !t || t->kernel != 0 || t->planes < 0 || t->block_waves < 0 ||
-        t->block_waves > 4 || t->nontemporal < 0 || t->nontemporal > 1)
+        t->block_waves > 4 || t->nontemporal < 0 || t->nontemporal > 1 || t->grid_blocks < 0)
         return fail(c, TLM_ERR_ARG, "tlm_set_tuning: bad arguments");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->tune = *t;

@@ -8,7 +8,11 @@
 //     p*P + k*Pp + j*nx + i. The slab padding is never read as a neighbour.
 //   * two such buffers (A = incident pulses of step n, B = of step n+1).
 //   * stubs: 3 planes of P floats, updated in place, dielectric cells only.
-//   * material: 1 byte per cell address.
+//   * cell code: 16 bits per cell address: material id in bits 0-7 (0 air,
+//     1 PEC, >= 2 dielectric), then one bit per face telling whether the
+//     neighbour across it is PEC (bit 8 x-, 9 x+, 10 y-, 11 y+, 12 z-,
+//     13 z+; 0 on outer faces). Built once at upload, so a cell needs a
+//     single 2-byte load and never reads its neighbours' materials.
 //
 // One step is (TlmSolver::step, TlmSolver.cpp:485-513):
 //   k_inject  : soft sources add 0.5*amp*sv(n) into A        (:487-505)
@@ -34,9 +38,9 @@ struct StepArgs
     const float *A;
     float *B;
     float *Vs;
-    const uint8_t *mat;
-    const float4 *props; // per dielectric id: (yh, gh, den, 0)
-    double *epart;       // energy partial per wave task
+    const uint16_t *code; // cell code (material + neighbour-PEC bits)
+    const float4 *props;  // per dielectric id: (yh, gh, den, 0)
+    double *epart;        // energy partial per wave task
     int nx, ny, nz;
     int nxy;             // nx*ny
     long long Pp;        // slab pitch: roundup(nx*ny, 64)
@@ -44,6 +48,7 @@ struct StepArgs
     int strips;          // 64-cell strips per slab: Pp/64
     int zc, nzc;         // planes per task, number of z chunks
     int tasks;           // strips * nzc
+    int groups;          // task groups (one per workgroup's worth of waves)
     float rho;
 };
 
@@ -61,20 +66,40 @@ template <bool NT> __device__ __forceinline__ void sto(float *base, unsigned bof
     else
         *p = v;
 }
-__device__ __forceinline__ unsigned ldm(const uint8_t *mat, unsigned off)
+__device__ __forceinline__ unsigned ldc(const uint16_t *code, unsigned off)
 {
-    return mat[off];
+    return code[off];
 }
-
-// Workgroups are dispatched to the 8 XCDs round robin; renumber them so each
-// XCD owns one contiguous range of wave tasks. Neighbouring strips then share
-// an L2, which keeps the cache lines written from both sides of a strip seam
-// (and the neighbour-row material bytes) on one die.
-__device__ __forceinline__ int xcd_block(int b, int G)
+enum : unsigned
 {
-    const int xcd = b % kNumXcd, local = b / kNumXcd;
-    const int per = G / kNumXcd, extra = G % kNumXcd;
-    return xcd * per + min(xcd, extra) + local;
+    NB_XM = 1u << 8,
+    NB_XP = 1u << 9,
+    NB_YM = 1u << 10,
+    NB_YP = 1u << 11,
+    NB_ZM = 1u << 12,
+    NB_ZP = 1u << 13
+};
+
+// Workgroups are dispatched to the 8 XCDs round robin. Each XCD gets one
+// contiguous range of task groups, walked by its own workgroups in a
+// grid-stride loop, so neighbouring strips share an L2 (which keeps the
+// cache lines written from both sides of a strip seam on one die). With as
+// many workgroups as groups this is a plain renumbering; with fewer it is a
+// persistent grid.
+struct XcdRange
+{
+    int first, end, step;
+};
+__device__ __forceinline__ XcdRange xcd_range(int b, int G, int ngroups)
+{
+    // ranges = min(8, G) so that every range has at least one workgroup
+    const int nr = min(kNumXcd, G);
+    const int xcd = b % nr, local = b / nr;
+    const int gper = G / nr + (xcd < G % nr ? 1 : 0); // workgroups on this range
+    const int per = ngroups / nr, extra = ngroups % nr;
+    const int start = xcd * per + min(xcd, extra);
+    const int count = per + (xcd < extra ? 1 : 0);
+    return {start + local, start + count, gper};
 }
 
 // Fused scatter + connect. A wavefront owns a strip of 64 consecutive cells
@@ -92,14 +117,10 @@ __device__ __forceinline__ int xcd_block(int b, int G)
 // The owner writes its own slot instead whenever the link is broken: own
 // cell PEC (value kept, times rho on an outer face), outer face (rho * out),
 // neighbour PEC (-out). This is connect(), TlmSolver.cpp:563-583.
+// One wave task: a 64-cell strip of slabs [k0, k1).
 template <bool STUBS, bool ENERGY, bool NT>
-__global__ void __launch_bounds__(256) k_step(StepArgs a)
+__device__ __forceinline__ void strip_task(const StepArgs &a, int task, int lane, double &esum)
 {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wpb = blockDim.x / kWave;
-    const int task = xcd_block(blockIdx.x, gridDim.x) * wpb + (threadIdx.x / kWave);
-    if (task >= a.tasks)
-        return; // whole wave
     const int tz = task / a.strips, t = task - tz * a.strips;
     const int f = t * kWave + lane; // cell inside the slab
     const bool act = f < a.nxy;
@@ -125,7 +146,6 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a)
 
     float zpo0 = 0.f, zpo1 = 0.f; // previous slab: reflected z+ pulses
     bool pecPrev = false;
-    double esum = 0.0;
     unsigned mo = (unsigned)(k0 * a.Pp + f); // cell index in the slab-padded space
     unsigned bo = mo * 4u;                   // its byte offset inside a plane
 
@@ -133,9 +153,11 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a)
     {
         float o[12];
         bool pec = true;
+        unsigned code = 0;
         if (act)
         {
-            const unsigned m = ldm(a.mat, mo);
+            code = ldc(a.code, mo);
+            const unsigned m = code & 0xFFu;
             pec = (m == 1u);
             float v[12];
 #pragma unroll
@@ -169,14 +191,11 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a)
         // every lane takes part in the shuffles
         const float L0 = __shfl_up(o[2], 1, kWave), L1 = __shfl_up(o[3], 1, kWave);
         const float R0 = __shfl_down(o[0], 1, kWave), R1 = __shfl_down(o[1], 1, kWave);
-        const int pecL_lane = __shfl_up((int)pec, 1, kWave);
-        const int pecR_lane = __shfl_down((int)pec, 1, kWave);
         if (act)
         {
             // ---- x faces: ports 0,1 (x-) and 2,3 (x+) ----
             {
-                const bool pecL = hasL ? (pecL_lane != 0) : (seamL ? ldm(a.mat, mo - 1) == 1u : true);
-                const bool pecR = hasR ? (pecR_lane != 0) : (seamR ? ldm(a.mat, mo + 1) == 1u : true);
+                const bool pecL = (code & NB_XM) != 0u, pecR = (code & NB_XP) != 0u;
 #pragma unroll
                 for (int pol = 0; pol < 2; ++pol)
                 {
@@ -213,8 +232,7 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a)
             }
             // ---- y faces: ports 4,5 (y-) and 6,7 (y+), sender push ----
             {
-                const bool pecLo = by0 ? false : ldm(a.mat, mo - (unsigned)a.nx) == 1u;
-                const bool pecHi = by1 ? false : ldm(a.mat, mo + (unsigned)a.nx) == 1u;
+                const bool pecLo = (code & NB_YM) != 0u, pecHi = (code & NB_YP) != 0u;
 #pragma unroll
                 for (int pol = 0; pol < 2; ++pol)
                 {
@@ -267,7 +285,7 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a)
                 }
                 else
                 {
-                    const bool pecLo = ldm(a.mat, mo - (unsigned)a.Pp) == 1u;
+                    const bool pecLo = (code & NB_ZM) != 0u;
                     if (pec)
                         sto<NT>(B[8 + pol], bo, ldo(A[8 + pol], bo));
                     else if (pecLo)
@@ -282,7 +300,7 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a)
                         sto<NT>(B[10 + pol], bo, rho * (pec ? ldo(A[10 + pol], bo) : oh));
                     else
                     {
-                        const bool pecHi = ldm(a.mat, mo + (unsigned)a.Pp) == 1u;
+                        const bool pecHi = (code & NB_ZP) != 0u;
                         if (pec)
                             sto<NT>(B[10 + pol], bo, ldo(A[10 + pol], bo));
                         else if (pecHi)
@@ -297,15 +315,31 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a)
         zpo1 = o[11];
         pecPrev = pec;
     }
+}
 
+template <bool STUBS, bool ENERGY, bool NT>
+__global__ void __launch_bounds__(256) k_step(StepArgs a)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wpb = blockDim.x / kWave;
+    const int wave = threadIdx.x / kWave;
+    const XcdRange gr = xcd_range(blockIdx.x, gridDim.x, a.groups);
+    double esum = 0.0;
+    for (int g = gr.first; g < gr.end; g += gr.step)
+    {
+        const int task = g * wpb + wave;
+        if (task < a.tasks)
+            strip_task<STUBS, ENERGY, NT>(a, task, lane, esum);
+    }
     if (ENERGY)
     {
         for (int off = kWave / 2; off > 0; off >>= 1)
             esum += __shfl_down(esum, off, kWave);
         if (lane == 0)
-            a.epart[task] = esum;
+            a.epart[blockIdx.x * wpb + wave] = esum;
     }
 }
+
 
 // Soft-source injection (TlmSolver.cpp:487-505). One thread per distinct
 // cell adds its sources in list order, so every slot sees the same

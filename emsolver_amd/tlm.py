@@ -235,15 +235,16 @@ class TlmDevice:
                                           C.byref(ker) if kernel else None), "tlm_time_steps")
         return tot.value, (ker.value if kernel else None)
 
-    def set_tuning(self, kernel=0, planes=0, block_waves=0, nontemporal=1):
-        t = _native.Tuning(int(kernel), int(planes), int(block_waves), int(nontemporal))
+    def set_tuning(self, kernel=0, planes=1, block_waves=0, nontemporal=0, grid_blocks=0):
+        t = _native.Tuning(int(kernel), int(planes), int(block_waves), int(nontemporal),
+                           int(grid_blocks))
         self._chk(self.lib.tlm_set_tuning(self.h, C.byref(t)), "tlm_set_tuning")
 
     def get_tuning(self):
         t = _native.Tuning()
         self._chk(self.lib.tlm_get_tuning(self.h, C.byref(t)), "tlm_get_tuning")
         return {"kernel": t.kernel, "planes": t.planes, "block_waves": t.block_waves,
-                "nontemporal": t.nontemporal}
+                "nontemporal": t.nontemporal, "grid_blocks": t.grid_blocks}
 
     def launch_shape(self):
         r, z, g, t = C.c_int(), C.c_int(), C.c_int(), C.c_int()

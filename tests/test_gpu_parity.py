@@ -115,10 +115,11 @@ def test_free_space_64_full_run_tolerances():
 
 
 TUNINGS = [
-    {"kernel": 0, "planes": 0, "block_waves": 0, "nontemporal": 1},
-    {"kernel": 0, "planes": 1, "block_waves": 1, "nontemporal": 0},
-    {"kernel": 0, "planes": 3, "block_waves": 2, "nontemporal": 1},
-    {"kernel": 0, "planes": 64, "block_waves": 4, "nontemporal": 0},
+    {"kernel": 0, "planes": 1, "block_waves": 4, "nontemporal": 0, "grid_blocks": 0},
+    {"kernel": 0, "planes": 0, "block_waves": 0, "nontemporal": 1, "grid_blocks": 0},
+    {"kernel": 0, "planes": 3, "block_waves": 1, "nontemporal": 0, "grid_blocks": 5},
+    {"kernel": 0, "planes": 1, "block_waves": 2, "nontemporal": 1, "grid_blocks": 17},
+    {"kernel": 0, "planes": 64, "block_waves": 4, "nontemporal": 0, "grid_blocks": 8},
 ]
 
 
@@ -131,7 +132,7 @@ def _run_tuned(sc, tuning, steps=None):
         return tlm.collect(dev, sc, steps)
 
 
-@pytest.mark.parametrize("tuning", TUNINGS, ids=lambda t: "z{planes}_w{block_waves}_nt{nontemporal}".format(**t))
+@pytest.mark.parametrize("tuning", TUNINGS, ids=lambda t: "z{planes}_w{block_waves}_nt{nontemporal}_g{grid_blocks}".format(**t))
 @pytest.mark.parametrize("shape", [(12, 9, 11), (300, 5, 7), (63, 7, 5), (2000, 2, 3)])
 def test_tunings_bitexact(shape, tuning):
     """Every launch shape gives the oracle's bits, including rows that start
