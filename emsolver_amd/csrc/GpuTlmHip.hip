@@ -48,7 +48,7 @@ struct tlm_ctx
     float4 *props = nullptr;
     int nmat = 0;
     bool stubs = false, uploaded = false;
-    double *epart = nullptr;
+    double *epart = nullptr, *epart2 = nullptr; // energy partials (per wave, per chunk)
     // sources
     long long *ucell = nullptr;
     int *ubeg = nullptr, *spol = nullptr;
@@ -295,8 +295,20 @@ int queue_steps(tlm_ctx *c, int step0, int nsteps, bool time_kernel, float *ms_k
         if (c->every > 0 && (n % c->every) == 0)
         {
             const int idx = n / c->every;
-            hipLaunchKernelGGL(tlm::k_energy_reduce, dim3(1), dim3(256), 0, c->stream, c->epart,
-                               (int)energy_parts(c), c->energy + idx);
+            const long long np = (long long)energy_parts(c);
+            if (np > 8192)
+            {
+                // two stages: 1024 fixed chunks, then one block
+                const long long chunk = (np + 1023) / 1024;
+                const int nb = (int)((np + chunk - 1) / chunk);
+                hipLaunchKernelGGL(tlm::k_energy_chunks, dim3(nb), dim3(256), 0, c->stream,
+                                   c->epart, np, chunk, c->epart2);
+                hipLaunchKernelGGL(tlm::k_energy_reduce, dim3(1), dim3(256), 0, c->stream,
+                                   c->epart2, nb, c->energy + idx);
+            }
+            else
+                hipLaunchKernelGGL(tlm::k_energy_reduce, dim3(1), dim3(256), 0, c->stream,
+                                   c->epart, (int)np, c->energy + idx);
             if ((int)c->energy_steps.size() <= idx)
                 c->energy_steps.resize(idx + 1, -1);
             c->energy_steps[idx] = n;
@@ -457,7 +469,8 @@ int tlm_create(const tlm_grid_desc *d, tlm_ctx **out)
     int rc;
     c->epart_cap = energy_parts(c);
     if ((rc = dalloc(c, &c->A, (size_t)c->P * 12)) || (rc = dalloc(c, &c->B, (size_t)c->P * 12)) ||
-        (rc = dalloc(c, &c->code, (size_t)c->P)) || (rc = dalloc(c, &c->epart, c->epart_cap)))
+        (rc = dalloc(c, &c->code, (size_t)c->P)) || (rc = dalloc(c, &c->epart, c->epart_cap)) ||
+        (rc = dalloc(c, &c->epart2, 1024)))
         return bail(rc);
     *out = c;
     return TLM_OK;

@@ -495,6 +495,27 @@ __global__ void k_monitor(MonitorArgs a)
     }
 }
 
+// First stage of the energy sum: block b adds partials [b*chunk, (b+1)*chunk)
+// in a fixed order (so the result does not depend on scheduling).
+__global__ void k_energy_chunks(const double *part, long long nparts, long long chunk, double *out)
+{
+    __shared__ double s[256];
+    const long long lo = blockIdx.x * chunk, hi = min(nparts, lo + chunk);
+    double acc = 0.0;
+    for (long long i = lo + threadIdx.x; i < hi; i += blockDim.x)
+        acc += part[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1)
+    {
+        if ((int)threadIdx.x < w)
+            s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        out[blockIdx.x] = s[0];
+}
+
 // Sum of the energy partials of one step in a fixed order.
 __global__ void k_energy_reduce(const double *part, int nparts, double *out)
 {
