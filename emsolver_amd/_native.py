@@ -70,8 +70,12 @@ SIGNATURES = {
     "tlm_build_huygens": (C.c_int64, [C.c_int, C.c_int, C.c_int, u8p, C.c_int, i64p, i32p,
                                       i32p, C.c_int64]),
     "tlm_dft_tables": (None, [dp, C.c_int, C.c_double, C.c_int, dp, dp]),
-    "tlm_schedule_tables": (None, [C.c_int, C.c_float, C.c_float, C.c_int, C.c_float, C.c_float,
-                                   fp, fp, fp]),
+    "tlm_schedule_tables": (None, [C.c_int, C.c_float, C.c_float, C.c_int, C.c_int, C.c_float,
+                                   C.c_float, fp, fp, fp]),
+    "tlm_set_plane": (C.c_int, [P, C.c_int, C.c_int]),
+    "tlm_set_frames": (C.c_int, [P, C.c_int, C.c_int]),
+    "tlm_read_plane": (C.c_int, [P, fp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "tlm_read_frames": (C.c_int, [P, i32p, fp, fp, C.c_int]),
 }
 
 

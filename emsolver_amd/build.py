@@ -3,8 +3,8 @@
     python -m emsolver_amd.build
 
 compiles ``emsolver_amd/lib/libemsolver_tlm.so`` from the HIP sources under
-``csrc/`` and the C++ drop-in glue under ``host/``, plus the host self-test
-program ``emsolver_amd/lib/tlm_selftest``.
+``csrc/`` and the C++ drop-in glue under ``host/`` (TlmSolver facade and
+RunGpuTlm), plus the self-test program ``tests/selftest/tlm_selftest``.
 
 Flags that matter:
   -ffp-contract=off  no fused multiply-add: every a*b+c rounds twice, as the
@@ -54,11 +54,13 @@ def build(verbose: bool = False) -> str:
     tmp = lib + ".tmp"
     _run([HIPCC, *DEVICE, *COMMON, "-shared", "-o", tmp, *srcs], verbose)
     os.replace(tmp, lib)
-    st = os.path.join(HERE, "host", "tlm_selftest.cpp")
+    # the self-test program (test infrastructure, lives with the tests)
+    st = os.path.join(ROOT, "tests", "selftest", "tlm_selftest.cpp")
     if os.path.exists(st):
-        exe = os.path.join(LIB_DIR, "tlm_selftest")
+        exe = os.path.join(ROOT, "tests", "selftest", "tlm_selftest")
         _run([HIPCC, *COMMON, "-o", exe + ".tmp", st, f"-L{LIB_DIR}", "-lemsolver_tlm",
-              "-Wl,-rpath,$ORIGIN", "-pthread"], verbose)
+              f"-Wl,-rpath,{LIB_DIR}", "-Wl,-rpath,$ORIGIN/../../emsolver_amd/lib", "-pthread",
+              "-ldl"], verbose)
         os.replace(exe + ".tmp", exe)
     return lib
 

@@ -77,6 +77,13 @@ struct tlm_ctx
     double *ctab = nullptr, *stab = nullptr, *dft_re = nullptr, *dft_im = nullptr;
     int nfreq = 0;
     int dft_done = 0; // probe DFT accumulated over steps [0, dft_done)
+    // cut plane and playback frames
+    int pa = -1, pidx = 0, pn1 = 0, pn2 = 0, plane_step = -1;
+    float *plane = nullptr;
+    int frame_every = 0, frame_max = 0;
+    float *frame_js = nullptr, *frame_plane = nullptr;
+    std::vector<int> frame_steps;
+    std::vector<int> frame_plane_step;
     // energy
     int every = 16;
     double *energy = nullptr;
@@ -237,7 +244,7 @@ int queue_steps(tlm_ctx *c, int step0, int nsteps, bool time_kernel, float *ms_k
     if (!c->sv || step0 + nsteps > c->nsteps)
         return fail(c, TLM_ERR_STATE, "step range exceeds the schedule set by tlm_set_schedule");
     HIPCHK(c, hipSetDevice(c->dev));
-    const int nmon = c->nf + c->nh + c->np + c->npr;
+    const int nmon0 = c->nf + c->nh + c->np + c->npr;
     if (time_kernel)
     {
         // events bracket every fused-kernel launch; no host wait inside the loop
@@ -256,9 +263,22 @@ int queue_steps(tlm_ctx *c, int step0, int nsteps, bool time_kernel, float *ms_k
             hipLaunchKernelGGL(tlm::k_inject, dim3(g), dim3(b), 0, c->stream, c->A, c->P, c->ucell,
                                c->ubeg, c->spol, c->samp, c->nuniq, c->sv, n);
         }
+        const bool plane_now = c->pa >= 0 && (n % 2) == 0;
+        const int pn = plane_now ? c->pn1 * c->pn2 : 0;
+        const int nmon = nmon0 + pn;
         if (nmon > 0)
         {
             tlm::MonitorArgs m;
+            m.pn = pn;
+            m.pa = c->pa;
+            m.pidx = c->pidx;
+            m.pn1 = c->pn1;
+            m.pn2 = c->pn2;
+            m.nx = c->nx;
+            m.ny = c->ny;
+            m.Pp = c->Pp;
+            m.code = c->code;
+            m.plane = c->plane;
             m.A = c->A;
             m.P = c->P;
             m.n = n;
@@ -286,6 +306,23 @@ int queue_steps(tlm_ctx *c, int step0, int nsteps, bool time_kernel, float *ms_k
             m.prrec = c->prrec;
             int b = 256, g = (nmon + b - 1) / b;
             hipLaunchKernelGGL(tlm::k_monitor, dim3(g), dim3(b), 0, c->stream, m);
+            if (plane_now)
+                c->plane_step = n;
+        }
+        if (c->frame_every > 0 && (n % c->frame_every) == 0 &&
+            (int)c->frame_steps.size() < c->frame_max)
+        {
+            // TlmSolver.cpp:747-763: the frame keeps |Js| of this step and the latest plane
+            const size_t f = c->frame_steps.size();
+            if (c->nf > 0)
+                HIPCHK(c, hipMemcpyAsync(c->frame_js + f * c->nf, c->js_inst, c->nf * sizeof(float),
+                                         hipMemcpyDeviceToDevice, c->stream));
+            const size_t np2 = (size_t)c->pn1 * c->pn2;
+            if (c->pa >= 0 && c->plane_step >= 0)
+                HIPCHK(c, hipMemcpyAsync(c->frame_plane + f * np2, c->plane, np2 * sizeof(float),
+                                         hipMemcpyDeviceToDevice, c->stream));
+            c->frame_steps.push_back(n);
+            c->frame_plane_step.push_back(c->pa >= 0 ? c->plane_step : -1);
         }
         if (time_kernel)
             HIPCHK(c, hipEventRecord(c->kev[2 * (n - step0)], c->stream));
@@ -411,7 +448,8 @@ const char *tlm_device_name(int device)
     hipDeviceProp_t p;
     if (hipGetDeviceProperties(&p, device) != hipSuccess)
         return "unknown adapter";
-    std::snprintf(name, sizeof(name), "%s (%s)", p.name, p.gcnArchName);
+    std::snprintf(name, sizeof(name), "%s (%s, %d CUs)", p.name[0] ? p.name : "AMD GPU",
+                  p.gcnArchName, p.multiProcessorCount);
     return name;
 }
 
@@ -668,6 +706,89 @@ int tlm_set_energy_every(tlm_ctx *c, int every)
     }
     c->energy_steps.clear();
     return TLM_OK;
+}
+
+int tlm_set_plane(tlm_ctx *c, int axis, int index)
+{
+    if (!c || axis > 2)
+        return fail(c, TLM_ERR_ARG, "tlm_set_plane: bad arguments");
+    dfree(c, c->plane);
+    c->pa = axis < 0 ? -1 : axis;
+    c->plane_step = -1;
+    if (c->pa < 0)
+        return TLM_OK;
+    const int nn[3] = {c->nx, c->ny, c->nz};
+    const int a1 = (axis == 0) ? 1 : 0, a2 = (axis == 2) ? 1 : 2;
+    c->pn1 = nn[a1];
+    c->pn2 = nn[a2];
+    c->pidx = std::min(std::max(index, 0), nn[axis] - 1);
+    int rc = dalloc(c, &c->plane, (size_t)c->pn1 * c->pn2);
+    if (rc)
+        return rc;
+    HIPCHK(c, hipMemset(c->plane, 0, (size_t)c->pn1 * c->pn2 * sizeof(float)));
+    return TLM_OK;
+}
+
+int tlm_set_frames(tlm_ctx *c, int every, int max_frames)
+{
+    if (!c || every < 0 || max_frames < 0)
+        return fail(c, TLM_ERR_ARG, "tlm_set_frames: bad arguments");
+    dfree(c, c->frame_js);
+    dfree(c, c->frame_plane);
+    c->frame_steps.clear();
+    c->frame_plane_step.clear();
+    c->frame_every = (max_frames > 0) ? every : 0;
+    c->frame_max = max_frames;
+    if (c->frame_every == 0)
+        return TLM_OK;
+    int rc;
+    if ((rc = dalloc(c, &c->frame_js, (size_t)max_frames * std::max(1, c->nf))))
+        return rc;
+    if (c->pa >= 0 &&
+        (rc = dalloc(c, &c->frame_plane, (size_t)max_frames * c->pn1 * c->pn2)))
+        return rc;
+    return TLM_OK;
+}
+
+int tlm_read_plane(tlm_ctx *c, float *out, int *n1, int *n2, int *step)
+{
+    if (!c)
+        return TLM_ERR_ARG;
+    if (n1)
+        *n1 = c->pa >= 0 ? c->pn1 : 0;
+    if (n2)
+        *n2 = c->pa >= 0 ? c->pn2 : 0;
+    if (step)
+        *step = c->pa >= 0 ? c->plane_step : -1;
+    if (out && c->pa >= 0)
+    {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpy(out, c->plane, (size_t)c->pn1 * c->pn2 * sizeof(float),
+                            hipMemcpyDeviceToHost));
+    }
+    return TLM_OK;
+}
+
+int tlm_read_frames(tlm_ctx *c, int32_t *steps, float *js, float *planes, int cap)
+{
+    if (!c || cap < 0)
+        return fail(c, TLM_ERR_ARG, "tlm_read_frames: bad arguments");
+    const int cnt = (int)c->frame_steps.size();
+    const int m = std::min(cap, cnt);
+    if (m > 0)
+    {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (steps)
+            for (int f = 0; f < m; ++f)
+                steps[f] = c->frame_steps[f];
+        if (js && c->nf > 0)
+            HIPCHK(c, hipMemcpy(js, c->frame_js, (size_t)m * c->nf * sizeof(float),
+                                hipMemcpyDeviceToHost));
+        if (planes && c->pa >= 0 && c->frame_plane)
+            HIPCHK(c, hipMemcpy(planes, c->frame_plane, (size_t)m * c->pn1 * c->pn2 * sizeof(float),
+                                hipMemcpyDeviceToHost));
+    }
+    return cnt;
 }
 
 int tlm_set_probes(tlm_ctx *c, const int64_t *cells, int n, float inv_dl)
@@ -1179,11 +1300,12 @@ int64_t tlm_build_huygens(int nx, int ny, int nz, const uint8_t *mat, int o, int
     return cnt;
 }
 
-void tlm_schedule_tables(int type, float f0, float dt, int nsteps, float gdw, float gdd, float *sv,
-                         float *cph, float *sph)
+void tlm_schedule_tables(int type, float f0, float dt, int step0, int nsteps, float gdw, float gdd,
+                         float *sv, float *cph, float *sph)
 {
-    for (int n = 0; n < nsteps; ++n)
+    for (int q = 0; q < nsteps; ++q)
     {
+        const int n = step0 + q;
         if (sv)
         {
             float val;
@@ -1215,13 +1337,13 @@ void tlm_schedule_tables(int type, float f0, float dt, int nsteps, float gdw, fl
                     val = std::exp(-u * u);
                 }
             }
-            sv[n] = val;
+            sv[q] = val;
         }
         float phase = 2.0f * (float)M_PI * f0 * (n * dt);
         if (cph)
-            cph[n] = std::cos(phase);
+            cph[q] = std::cos(phase);
         if (sph)
-            sph[n] = std::sin(phase);
+            sph[q] = std::sin(phase);
     }
 }
 

@@ -137,11 +137,9 @@ __device__ __forceinline__ void strip_task(const StepArgs &a, int task, int lane
         A[p] = a.A + p * a.P;
         B[p] = a.B + p * a.P;
     }
-    // x neighbour inside the wave, or across a strip seam
+    // x neighbour inside the wave; otherwise (strip seam) the two cells push
     const bool hasL = act && lane > 0 && i > 0;
-    const bool seamL = act && lane == 0 && i > 0;
     const bool hasR = act && lane < kWave - 1 && i < a.nx - 1;
-    const bool seamR = act && lane == kWave - 1 && i < a.nx - 1;
     const bool bx0 = i == 0, bx1 = i == a.nx - 1, by0 = j == 0, by1 = j == a.ny - 1;
 
     float zpo0 = 0.f, zpo1 = 0.f; // previous slab: reflected z+ pulses
@@ -401,6 +399,12 @@ struct MonitorArgs
     const long long *prcell;
     float inv_dl;
     float *prrec; // [step][npr][3]
+    // |E| cut plane (0 threads on steps that do not take it)
+    int pn, pa, pidx, pn1, pn2;
+    int nx, ny;
+    long long Pp;
+    const uint16_t *code;
+    float *plane;
 };
 
 // Monitors on the post-injection incident pulses (TlmSolver::monitors,
@@ -492,6 +496,33 @@ __global__ void k_monitor(MonitorArgs a)
         rec[0] = a.inv_dl * scn::vj(v, 0);
         rec[1] = a.inv_dl * scn::vj(v, 1);
         rec[2] = a.inv_dl * scn::vj(v, 2);
+        return;
+    }
+    id -= a.npr;
+    if (id < a.pn)
+    {
+        // |E| = sqrt(sum_j V_j^2) on the plane, PEC cells 0 (TlmSolver.cpp:715-745)
+        const int q1 = id % a.pn1, q2 = id / a.pn1;
+        const int a1 = (a.pa == 0) ? 1 : 0, a2 = (a.pa == 2) ? 1 : 2;
+        int co[3];
+        co[a.pa] = a.pidx;
+        co[a1] = q1;
+        co[a2] = q2;
+        const long long c = (long long)co[2] * a.Pp + (long long)co[1] * a.nx + co[0];
+        float e = 0.0f;
+        if ((a.code[c] & 0xFFu) != 1u)
+        {
+            float v[12];
+            gather12(a.A, a.P, c, v);
+            float e2 = 0.0f;
+            for (int j = 0; j < 3; ++j)
+            {
+                const float vj = scn::vj(v, j);
+                e2 += vj * vj;
+            }
+            e = sqrtf(e2);
+        }
+        a.plane[id] = e;
     }
 }
 

@@ -57,6 +57,9 @@ class Scene:
     probes: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0, np.int64))
     dft_freqs: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0))
     energy_every: int = 16
+    plane_axis: int = -1          # |E| cut plane (TlmSolver::setFieldPlane), -1 off
+    plane_index: int = 0
+    record_every: int = 0         # playback frames (TlmConfig::recordEvery), 0 off
     faces: bool = True            # PEC surface-current monitor (always on in the reference)
     V0: np.ndarray | None = None  # optional initial pulses (12 per cell, AoS)
     Vs0: np.ndarray | None = None

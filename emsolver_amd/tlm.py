@@ -62,7 +62,7 @@ def schedule_tables(waveform: int, f0: float, dt: float, nsteps: int, gd_width=2
     sv = np.empty(nsteps, np.float32)
     cph = np.empty(nsteps, np.float32)
     sph = np.empty(nsteps, np.float32)
-    lib.tlm_schedule_tables(waveform, f0, float(dt), nsteps, gd_width, gd_delay,
+    lib.tlm_schedule_tables(waveform, f0, float(dt), 0, nsteps, gd_width, gd_delay,
                             ptr(sv, C.c_float), ptr(cph, C.c_float), ptr(sph, C.c_float))
     return sv, cph, sph
 
@@ -218,6 +218,36 @@ class TlmDevice:
                   "tlm_set_huygens")
         self.nhuy = c.size
 
+    def set_plane(self, axis, index):
+        self._chk(self.lib.tlm_set_plane(self.h, int(axis), int(index)), "tlm_set_plane")
+        self.plane_axis = int(axis)
+
+    def set_frames(self, every, max_frames):
+        self._chk(self.lib.tlm_set_frames(self.h, int(every), int(max_frames)), "tlm_set_frames")
+
+    def read_plane(self):
+        n1, n2, st = C.c_int(), C.c_int(), C.c_int()
+        self._chk(self.lib.tlm_read_plane(self.h, None, C.byref(n1), C.byref(n2), C.byref(st)),
+                  "tlm_read_plane")
+        out = np.empty((n2.value, n1.value), np.float32)
+        if out.size:
+            self._chk(self.lib.tlm_read_plane(self.h, ptr(out, C.c_float), C.byref(n1),
+                                              C.byref(n2), C.byref(st)), "tlm_read_plane")
+        return out, st.value
+
+    def read_frames(self):
+        cnt = self.lib.tlm_read_frames(self.h, None, None, None, 0)
+        if cnt < 0:
+            self._chk(cnt, "tlm_read_frames")
+        n1, n2, st = C.c_int(), C.c_int(), C.c_int()
+        self.lib.tlm_read_plane(self.h, None, C.byref(n1), C.byref(n2), C.byref(st))
+        steps = np.empty(cnt, np.int32)
+        js = np.empty((cnt, self.nface), np.float32)
+        planes = np.empty((cnt, n2.value, n1.value), np.float32) if n1.value else None
+        self.lib.tlm_read_frames(self.h, ptr(steps, C.c_int32), ptr(js, C.c_float),
+                                 ptr(planes, C.c_float), cnt)
+        return steps, js, planes
+
     def set_energy_every(self, every):
         self._chk(self.lib.tlm_set_energy_every(self.h, int(every)), "tlm_set_energy_every")
 
@@ -317,6 +347,9 @@ class SceneResult:
     energy: np.ndarray | None = None
     V: np.ndarray | None = None
     Vs: np.ndarray | None = None
+    plane: np.ndarray | None = None
+    plane_step: int = -1
+    frames: tuple | None = None            # (steps, js [f][nface], planes [f][n2][n1])
     kernel_ms: float | None = None
     total_ms: float | None = None
 
@@ -363,6 +396,10 @@ def open_scene(scene, device=0) -> TlmDevice:
         if len(scene.dft_freqs):
             ct, st = dft_tables(scene.dft_freqs, float(dt), total)
             dev.set_probe_dft(ct, st)
+    if scene.plane_axis >= 0:
+        dev.set_plane(scene.plane_axis, scene.plane_index)
+    if scene.record_every > 0:
+        dev.set_frames(scene.record_every, total // scene.record_every + 1)
     dev.dt = dt
     return dev
 
@@ -382,6 +419,10 @@ def collect(dev: TlmDevice, scene, steps: int, state=True) -> SceneResult:
         res.huygens = dev.read_huygens()
         res.huy_samples = res.dft_samples
     res.energy_steps, res.energy = dev.read_energy()
+    if scene.plane_axis >= 0:
+        res.plane, res.plane_step = dev.read_plane()
+    if scene.record_every > 0:
+        res.frames = dev.read_frames()
     if state:
         res.V, res.Vs = dev.read_state()
     return res

@@ -119,6 +119,16 @@ int tlm_set_huygens(tlm_ctx *ctx, const int64_t *cells, const int32_t *axes, int
  * 0 disables. */
 int tlm_set_energy_every(tlm_ctx *ctx, int every);
 
+/* |E| cut plane normal to `axis` at `index` (clamped into the grid), taken
+ * on the post-injection pulses of every even step, PEC cells 0
+ * (TlmSolver.cpp:715-745). axis < 0 disables. */
+int tlm_set_plane(tlm_ctx *ctx, int axis, int index);
+
+/* Playback frames every `every` steps (TlmSolver.cpp:747-763): each holds
+ * the instantaneous |Js| of every face and the latest cut plane. At most
+ * max_frames are kept; 0 disables. */
+int tlm_set_frames(tlm_ctx *ctx, int every, int max_frames);
+
 /* Queues steps [step0, step0 + nsteps): inject -> monitors -> fused
  * scatter/connect (TlmSolver::step, TlmSolver.cpp:485-513). Asynchronous. */
 int tlm_run(tlm_ctx *ctx, int step0, int nsteps);
@@ -150,6 +160,14 @@ int tlm_read_huygens(tlm_ctx *ctx, float *dft8);
 /* Energy samples recorded so far (steps and FP64 sums); returns the count,
  * writing at most cap entries. */
 int tlm_read_energy(tlm_ctx *ctx, int32_t *steps, double *vals, int cap);
+
+/* Latest cut plane: n1*n2 floats (row q2, column q1 as planeBuf,
+ * TlmSolver.cpp:722-743). *step is the step it was taken at, -1 if none. */
+int tlm_read_plane(tlm_ctx *ctx, float *out, int *n1, int *n2, int *step);
+
+/* Recorded frames: returns the count; fills at most cap frames: steps[f],
+ * js[f * nfaces ...], planes[f * n1*n2 ...] (either array may be NULL). */
+int tlm_read_frames(tlm_ctx *ctx, int32_t *steps, float *js, float *planes, int cap);
 
 /* Runs steps [step0, step0+nsteps) and measures them with HIP events on the
  * context stream: *ms_total spans the whole queue of the steps, *ms_kernel
@@ -209,9 +227,9 @@ enum { TLM_WF_CW_RAMPED = 0, TLM_WF_GAUSS_SINE = 1, TLM_WF_GAUSS_PULSE = 2, TLM_
 
 /* Per-step host scalars for tlm_set_schedule: sv(n) = waveformValue(n)
  * (TlmSolver.cpp:410-436) and cph/sph of the f0 phase 2*pi*f0*(n*dt)
- * (TlmSolver.cpp:619-620), all in float like the reference. Any output
- * pointer may be NULL. */
-void tlm_schedule_tables(int waveform, float f0, float dt, int nsteps, float gd_width,
+ * (TlmSolver.cpp:619-620), all in float like the reference, for steps
+ * step0 .. step0+nsteps-1 (out[n - step0]). Any output pointer may be NULL. */
+void tlm_schedule_tables(int waveform, float f0, float dt, int step0, int nsteps, float gd_width,
                          float gd_delay, float *sv, float *cph, float *sph);
 
 #ifdef __cplusplus

@@ -316,6 +316,23 @@ class OracleTlm:
         self.lib.ot_plane(self.h, _ptr(b, C.c_double), C.byref(n1), C.byref(n2))
         return b.reshape(n2.value, n1.value) if n else b
 
+    def frames_full(self):
+        """(steps, js [f][nface], planes [f][n2][n1] or None)."""
+        nf = self.lib.ot_num_faces(self.h)
+        cnt = self.lib.ot_num_frames(self.h)
+        n1 = C.c_int(0)
+        n2 = C.c_int(0)
+        self.lib.ot_plane(self.h, None, C.byref(n1), C.byref(n2))
+        steps = np.empty(cnt, np.int32)
+        js = np.empty((cnt, nf))
+        planes = np.empty((cnt, n2.value, n1.value)) if n1.value else None
+        for i in range(cnt):
+            pl = np.empty(n1.value * n2.value) if planes is not None else None
+            steps[i] = self.lib.ot_frame(self.h, i, _ptr(js[i], C.c_double), _ptr(pl, C.c_double))
+            if planes is not None:
+                planes[i] = pl.reshape(n2.value, n1.value)
+        return steps, js, planes
+
     def frames(self):
         out = []
         nf = self.lib.ot_num_faces(self.h)

@@ -15,9 +15,12 @@ def oracle_open(scene, fp64=False, threads=0) -> OracleTlm:
     cfg = OracleConfig(scene.nx, scene.ny, scene.nz, dl=scene.dl, origin=scene.origin,
                        f0=scene.f0, waveform=scene.waveform, total_steps=scene.total_steps,
                        settle_steps=scene.settle_steps, rho=scene.rho,
-                       huy_offset=scene.huy_offset, gd_width=scene.gd_width,
-                       gd_delay=scene.gd_delay, threads=threads, fp64=fp64)
+                       huy_offset=scene.huy_offset, record_every=scene.record_every,
+                       gd_width=scene.gd_width, gd_delay=scene.gd_delay, threads=threads,
+                       fp64=fp64)
     o = OracleTlm(cfg, scene.mat, scene.epsr, scene.sigma)
+    if scene.plane_axis >= 0:
+        o.set_field_plane(scene.plane_axis, scene.plane_index)
     for c, p, a in scene.sources:
         o.add_source(c, p, a)
     for cells, p, a in scene.ports:
@@ -53,6 +56,11 @@ def oracle_collect(o: OracleTlm, scene) -> SceneResult:
         res.huy_samples = hs
     st, _, ed = o.energy()
     res.energy_steps, res.energy = st, ed
+    if scene.plane_axis >= 0:
+        res.plane = o.plane()
+    if scene.record_every > 0:
+        fr = o.frames_full()
+        res.frames = fr
     res.V, res.Vs = o.state()
     return res
 
@@ -108,6 +116,15 @@ def compare_results(gpu: SceneResult, ref: SceneResult, scene, state=True, energ
         assert gpu.dft_samples == ref.dft_samples
     if scene.huy_offset >= 2 and gpu.huygens is not None:
         assert_bitexact("huygens", gpu.huygens, ref.huygens)
+    if scene.plane_axis >= 0:
+        assert_bitexact("plane", gpu.plane, ref.plane)
+    if scene.record_every > 0:
+        gs, gj, gp = gpu.frames
+        rs, rj, rp = ref.frames
+        assert np.array_equal(gs, rs), "frame steps"
+        assert_bitexact("frame js", gj, rj)
+        if gp is not None:
+            assert_bitexact("frame planes", gp, rp)
     assert np.array_equal(gpu.energy_steps, ref.energy_steps), "energy sample steps"
     if len(ref.energy):
         rel = np.abs(gpu.energy - ref.energy) / np.maximum(np.abs(ref.energy), 1e-300)

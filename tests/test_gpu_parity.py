@@ -143,3 +143,13 @@ def test_tunings_bitexact(shape, tuning):
     g = _run_tuned(sc, tuning)
     o = helpers.oracle_run(sc)
     helpers.compare_results(g, o, sc)
+
+
+def test_plane_and_frames_bitexact():
+    """|E| cut plane (every even step) and playback frames (|Js| + latest
+    plane every record_every steps) match the reference semantics."""
+    sc = scenes.dielectric_boxes(n=30, nboxes=4, steps=47, nprobes=3, nfreq=2, seed=9)
+    sc.plane_axis, sc.plane_index, sc.record_every = 1, 13, 5
+    _check(sc)
+    sc.plane_axis, sc.plane_index = 0, 99   # clamped to the last x plane
+    _check(sc.with_steps(70))
