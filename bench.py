@@ -41,9 +41,33 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def alg_bytes_per_cell(scene, pulse_bytes=4, stub_bytes=4):
-    """B_alg = 2*12*s_p + 1 + 2*3*s_s*f_diel (material id 1 B)."""
-    return 2 * 12 * pulse_bytes + 1 + 2 * 3 * stub_bytes * scene.diel_fraction()
+PULSE_BYTES = {"f32": 4, "f16": 2, "bf16": 2}
+
+
+def alg_bytes_per_cell(scene, stub_bytes=4):
+    """B_alg = 2*12*s_p + 1 + 2*3*s_s*f_diel (material id 1 B). For the 16-bit
+    formats s_p is counted per slab: 4 B in the FP32 slabs near sources, 2 B
+    elsewhere."""
+    sp = PULSE_BYTES[scene.pulse]
+    if scene.pulse != "f32":
+        sp = 2 + 2 * fp32_slab_fraction(scene)
+    return 2 * 12 * sp + 1 + 2 * 3 * stub_bytes * scene.diel_fraction()
+
+
+def fp32_slab_fraction(scene):
+    """Fraction of z-slabs a 16-bit run keeps in FP32 (within fp32_halo of a
+    soft-source or port slab)."""
+    if scene.pulse == "f32":
+        return 1.0
+    if scene.fp32_halo < 0:
+        return 0.0
+    slab = scene.nx * scene.ny
+    hot = np.zeros(scene.nz, bool)
+    cells = [c for c, _, _ in scene.sources] + [c for pc, _, _ in scene.ports for c in pc]
+    for c in cells:
+        k = int(c) // slab
+        hot[max(0, k - scene.fp32_halo):k + scene.fp32_halo + 1] = True
+    return float(hot.mean())
 
 
 def read_traffic(workload):
@@ -121,6 +145,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the 512^3 free-space line")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--extra-steps", type=int, default=200,
+                    help="timed steps of each extra 512^3 line")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -154,15 +180,32 @@ def main():
         extra["device_copy_gbs"] = round(gbps.value, 1)
         log(f"[bench] device copy {gbps.value:.0f} GB/s")
     if not args.no_extra:
-        big = scenes.free_space_cube(n=512, steps=200, probe_offset=10)
-        r = gpu_run(big, 10, 200, "free_space_512")
+        for pulse in ("f32", "f16", "bf16"):
+            sc16 = scenes.dielectric_boxes(n=512, nboxes=64, steps=args.extra_steps + 10,
+                                           nprobes=16, nfreq=32, seed=1)
+            sc16.pulse = pulse
+            r = gpu_run(sc16, 10, args.extra_steps, f"diel_boxes_512_{pulse}")
+            extra[f"diel_boxes_512_{pulse}"] = {
+                "gcell_s": round(r["gcells"], 3),
+                "kernel_ms_avg": round(r["kernel_ms_avg"], 4),
+                "achieved_gbs": round(r["achieved_gbs"], 1),
+                "roofline_frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                "alg_bytes_per_cell": round(r["alg_bytes_per_cell"], 3),
+                "fp32_slab_fraction": round(fp32_slab_fraction(sc16), 4),
+                "steps": args.extra_steps,
+            }
+            log(f"[bench] diel_boxes_512_{pulse}: {r['gcells']:.2f} Gcell/s, kernel "
+                f"{r['kernel_ms_avg']:.3f} ms, {r['achieved_gbs']:.0f} GB/s = "
+                f"{r['achieved_gbs'] / HBM_PEAK_GBS:.3f} of peak")
+        big = scenes.free_space_cube(n=512, steps=args.extra_steps + 10, probe_offset=10)
+        r = gpu_run(big, 10, args.extra_steps, "free_space_512")
         extra["free_space_512_f32"] = {
             "gcell_s": round(r["gcells"], 3),
             "kernel_ms_avg": round(r["kernel_ms_avg"], 4),
             "achieved_gbs": round(r["achieved_gbs"], 1),
             "roofline_frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
             "alg_bytes_per_cell": r["alg_bytes_per_cell"],
-            "steps": 200,
+            "steps": args.extra_steps,
         }
         log(f"[bench] free_space_512: {r['gcells']:.2f} Gcell/s, kernel {r['kernel_ms_avg']:.3f} ms, "
             f"{r['achieved_gbs']:.0f} GB/s = {r['achieved_gbs'] / HBM_PEAK_GBS:.3f} of peak")

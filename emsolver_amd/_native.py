@@ -25,7 +25,12 @@ u8p = C.POINTER(C.c_uint8)
 class GridDesc(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("dl", C.c_float),
                 ("rho", C.c_float), ("pulse_type", C.c_int32), ("device", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("fp32_halo", C.c_int32)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("planes", C.c_int32), ("block_waves", C.c_int32), ("grid_blocks", C.c_int32),
+                ("graph_ticks", C.c_int32)]
 
 
 class Tuning(C.Structure):
@@ -76,6 +81,23 @@ SIGNATURES = {
     "tlm_set_frames": (C.c_int, [P, C.c_int, C.c_int]),
     "tlm_read_plane": (C.c_int, [P, fp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "tlm_read_frames": (C.c_int, [P, i32p, fp, fp, C.c_int]),
+    "tlm_read_scales": (C.c_int, [P, fp, fp, i32p]),
+    "tlm_batch_create": (C.c_int, [C.c_int, C.POINTER(P)]),
+    "tlm_batch_destroy": (None, [P]),
+    "tlm_batch_last_error": (C.c_char_p, [P]),
+    "tlm_batch_add": (C.c_int, [P, P, C.c_int]),
+    "tlm_batch_remove": (C.c_int, [P, P]),
+    "tlm_batch_run": (C.c_int, [P, C.c_int]),
+    "tlm_batch_sync": (C.c_int, [P]),
+    "tlm_batch_members": (C.c_int, [P, C.POINTER(P), i32p, C.c_int]),
+    "tlm_batch_set_tuning": (C.c_int, [P, C.POINTER(BatchTuning)]),
+    "tlm_batch_get_tuning": (C.c_int, [P, C.POINTER(BatchTuning)]),
+    "tlm_batch_time": (C.c_int, [P, C.c_int, fp, i64p]),
+    "tlm_batch_autotune": (C.c_int, [P, C.c_int, C.POINTER(BatchTuning)]),
+    "tlm_device_memory": (C.c_int, [C.c_int, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "tlm_estimate_bytes": (C.c_int64, [C.POINTER(GridDesc), C.c_int]),
+    "tlm_device_encode": (C.c_int, [C.c_int, C.c_int, fp, C.c_float, C.POINTER(C.c_uint16),
+                                    C.c_int64]),
 }
 
 

@@ -4,8 +4,7 @@
 // (GpuTlm.cpp:532-884 in the reference) with HIP: one stream per context,
 // device-resident per-step tables, three launches per step (inject,
 // monitors, fused scatter/connect) and no host synchronisation inside a run.
-#include "../../include/GpuTlmHip.h"
-#include "tlm_kernels.h"
+#include "tlm_context.h"
 
 #include <hip/hip_runtime.h>
 
@@ -29,71 +28,8 @@ struct DevBuf
 
 } // namespace
 
-struct tlm_ctx
-{
-    int dev = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, evk0 = nullptr, evk1 = nullptr;
-    std::vector<hipEvent_t> kev; // per-launch timing events (2 per step), created on demand
-    int nx = 0, ny = 0, nz = 0;
-    long long nxy = 0, Pp = 0, P = 0; // slab pitch roundup(nx*ny, 64), plane stride Pp*nz
-    float dl = 0.f, rho = 0.f;
-    // launch shape of the fused kernel
-    tlm_tuning tune = {0, 1, 4, 0, 0};
-    int strips = 0, zc = 0, nzc = 0, tasks = 0, groups = 0, blocks = 0, wpb = 4;
-    size_t epart_cap = 0;
-    // state
-    float *A = nullptr, *B = nullptr, *Vs = nullptr;
-    uint16_t *code = nullptr; // cell code: material + neighbour-PEC bits
-    float4 *props = nullptr;
-    int nmat = 0;
-    bool stubs = false, uploaded = false;
-    double *epart = nullptr, *epart2 = nullptr; // energy partials (per wave, per chunk)
-    // sources
-    long long *ucell = nullptr;
-    int *ubeg = nullptr, *spol = nullptr;
-    float *samp = nullptr;
-    int nuniq = 0;
-    // schedule
-    float *sv = nullptr, *cph = nullptr, *sph = nullptr;
-    int nsteps = 0, settle = 0;
-    // monitors
-    long long *fcell = nullptr;
-    int *faxis = nullptr;
-    float *js_inst = nullptr, *js_dft4 = nullptr;
-    int nf = 0;
-    long long *hcell = nullptr;
-    int *haxis = nullptr;
-    float *huy8 = nullptr;
-    int nh = 0;
-    long long *pcells = nullptr, *ploop = nullptr;
-    int *poff = nullptr, *ppol = nullptr;
-    float *prec = nullptr;
-    int np = 0;
-    long long *prcell = nullptr;
-    float inv_dl = 0.f;
-    float *prrec = nullptr;
-    int npr = 0;
-    double *ctab = nullptr, *stab = nullptr, *dft_re = nullptr, *dft_im = nullptr;
-    int nfreq = 0;
-    int dft_done = 0; // probe DFT accumulated over steps [0, dft_done)
-    // cut plane and playback frames
-    int pa = -1, pidx = 0, pn1 = 0, pn2 = 0, plane_step = -1;
-    float *plane = nullptr;
-    int frame_every = 0, frame_max = 0;
-    float *frame_js = nullptr, *frame_plane = nullptr;
-    std::vector<int> frame_steps;
-    std::vector<int> frame_plane_step;
-    // energy
-    int every = 16;
-    double *energy = nullptr;
-    std::vector<int> energy_steps;
-    int steps_run_hi = 0; // highest step index + 1 queued so far
-    std::string err = "no error";
-    std::vector<void *> owned;
-};
 
-namespace {
+namespace tlmi {
 
 int fail(tlm_ctx *c, int code, const std::string &msg)
 {
@@ -104,68 +40,7 @@ int fail(tlm_ctx *c, int code, const std::string &msg)
     return code;
 }
 
-int hip_fail(tlm_ctx *c, hipError_t e, const char *what)
-{
-    std::string m = std::string(what) + ": " + hipGetErrorString(e);
-    return fail(c, e == hipErrorOutOfMemory ? TLM_ERR_NOMEM : TLM_ERR_HIP, m);
-}
 
-#define HIPCHK(ctx, call)                                                                          \
-    do                                                                                             \
-    {                                                                                              \
-        hipError_t e_ = (call);                                                                    \
-        if (e_ != hipSuccess)                                                                      \
-            return hip_fail((ctx), e_, #call);                                                     \
-    } while (0)
-
-template <class T> int dalloc(tlm_ctx *c, T **p, size_t count)
-{
-    *p = nullptr;
-    if (count == 0)
-        count = 1;
-    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
-    if (e != hipSuccess)
-    {
-        *p = nullptr;
-        return hip_fail(c, e, "hipMalloc");
-    }
-    c->owned.push_back(*p);
-    return TLM_OK;
-}
-
-template <class T> void dfree(tlm_ctx *c, T *&p)
-{
-    if (!p)
-        return;
-    auto it = std::find(c->owned.begin(), c->owned.end(), (void *)p);
-    if (it != c->owned.end())
-        c->owned.erase(it);
-    (void)hipFree(p);
-    p = nullptr;
-}
-
-template <class T> int upload(tlm_ctx *c, T **dst, const T *src, size_t count)
-{
-    dfree(c, *dst);
-    int rc = dalloc(c, dst, count);
-    if (rc)
-        return rc;
-    if (count && src)
-        HIPCHK(c, hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice));
-    return TLM_OK;
-}
-
-// Device address of cell (k*ny + j)*nx + i: k*Pp + j*nx + i.
-long long padded_cell(const tlm_ctx *c, long long cell)
-{
-    long long k = cell / c->nxy;
-    return k * c->Pp + (cell - k * c->nxy);
-}
-
-bool cell_ok(const tlm_ctx *c, long long cell)
-{
-    return cell >= 0 && cell < (long long)c->nx * c->ny * c->nz;
-}
 
 // Launch shape: a wave task is one 64-cell strip of a z-slab marched over
 // zc slabs (tuning.planes; 0 picks zc for about 64K tasks). Task groups of
@@ -194,15 +69,24 @@ size_t energy_parts(const tlm_ctx *c)
     return (size_t)c->blocks * c->wpb;
 }
 
-template <bool S, bool E>
+template <bool S, bool E, int PF>
 void launch_fused(tlm_ctx *c, const tlm::StepArgs &a)
 {
     dim3 grid(c->blocks), block(c->wpb * tlm::kWave);
-    if (c->tune.nontemporal)
-        hipLaunchKernelGGL((tlm::k_step<S, E, true>), grid, block, 0, c->stream, a);
+    if (c->tune.nontemporal && PF == tlm::PF_F32)
+        hipLaunchKernelGGL((tlm::k_step<S, E, true, PF>), grid, block, 0, c->stream, a);
     else
-        hipLaunchKernelGGL((tlm::k_step<S, E, false>), grid, block, 0, c->stream, a);
+        hipLaunchKernelGGL((tlm::k_step<S, E, false, PF>), grid, block, 0, c->stream, a);
 }
+
+template <int PF> void launch_pf(tlm_ctx *c, const tlm::StepArgs &a, bool energy)
+{
+    if (c->stubs)
+        energy ? launch_fused<true, true, PF>(c, a) : launch_fused<true, false, PF>(c, a);
+    else
+        energy ? launch_fused<false, true, PF>(c, a) : launch_fused<false, false, PF>(c, a);
+}
+
 
 int launch_step(tlm_ctx *c, int n)
 {
@@ -214,27 +98,75 @@ int launch_step(tlm_ctx *c, int n)
     a.code = c->code;
     a.props = c->props;
     a.epart = c->epart;
+    a.slab = c->slab;
+    a.dsc = nullptr;
+    a.esc = nullptr;
+    a.mpart = c->mpart;
+    if (c->pf != TLM_PULSE_F32)
+    {
+        const int cur = scale_buf(c, n);
+        a.dsc = c->scl[cur];
+        a.esc = c->iscl[cur ^ 1];
+    }
     a.nx = c->nx;
     a.ny = c->ny;
     a.nz = c->nz;
     a.nxy = (int)c->nxy;
     a.Pp = c->Pp;
     a.P = c->P;
+    a.PB = c->PB;
     a.strips = c->strips;
     a.zc = c->zc;
     a.nzc = c->nzc;
     a.tasks = c->tasks;
     a.groups = c->groups;
     a.rho = c->rho;
-    if (c->stubs)
-        energy ? launch_fused<true, true>(c, a) : launch_fused<true, false>(c, a);
+    if (c->pf == TLM_PULSE_F16)
+        launch_pf<tlm::PF_F16>(c, a, energy);
+    else if (c->pf == TLM_PULSE_BF16)
+        launch_pf<tlm::PF_BF16>(c, a, energy);
     else
-        energy ? launch_fused<false, true>(c, a) : launch_fused<false, false>(c, a);
+        launch_pf<tlm::PF_F32>(c, a, energy);
+    if (c->pf != TLM_PULSE_F32)
+        hipLaunchKernelGGL(tlm::k_slab_max, dim3(c->nz), dim3(256), 0, c->stream, c->mpart,
+                           c->strips, c->mreg);
     return TLM_OK;
 }
 
+void queue_probe_dft(tlm_ctx *c, int n0, int n1, hipStream_t s)
+{
+    if (c->npr <= 0 || c->nfreq <= 0 || n1 <= n0)
+        return;
+    const int tot = c->npr * 3 * c->nfreq, b = 128;
+    hipLaunchKernelGGL(tlm::k_probe_dft, dim3((tot + b - 1) / b), dim3(b), 0, s, c->prrec, c->npr,
+                       c->ctab, c->stab, c->nfreq, n0, n1, c->dft_re, c->dft_im);
+    c->dft_done = n1;
+}
+
+tlm::PulseView pulse_view(const tlm_ctx *c, int n)
+{
+    tlm::PulseView v;
+    v.base = c->A;
+    v.PB = c->PB;
+    v.Pp = c->Pp;
+    v.pf = c->pf;
+    v.slab = c->slab;
+    v.sc = nullptr;
+    v.isc = nullptr;
+    if (c->pf != TLM_PULSE_F32)
+    {
+        const int cur = scale_buf(c, n);
+        v.sc = c->scl[cur];
+        v.isc = c->iscl[cur];
+    }
+    return v;
+}
+
+
 int queue_steps(tlm_ctx *c, int step0, int nsteps, bool time_kernel, float *ms_kernel)
 {
+    if (c->batch)
+        return fail(c, TLM_ERR_STATE, "context is a batch member (tlm_batch_remove it first)");
     if (!c->uploaded)
         return fail(c, TLM_ERR_STATE, "tlm_run before tlm_upload");
     if (nsteps < 0 || step0 < 0)
@@ -244,6 +176,17 @@ int queue_steps(tlm_ctx *c, int step0, int nsteps, bool time_kernel, float *ms_k
     if (!c->sv || step0 + nsteps > c->nsteps)
         return fail(c, TLM_ERR_STATE, "step range exceeds the schedule set by tlm_set_schedule");
     HIPCHK(c, hipSetDevice(c->dev));
+    if (c->pf != TLM_PULSE_F32)
+    {
+        if (!c->materialized)
+        {
+            int rc = materialize(c, step0);
+            if (rc)
+                return rc;
+        }
+        else if (step0 != c->next_step)
+            return fail(c, TLM_ERR_STATE, "16-bit pulse storage runs contiguous step blocks");
+    }
     const int nmon0 = c->nf + c->nh + c->np + c->npr;
     if (time_kernel)
     {
@@ -257,11 +200,30 @@ int queue_steps(tlm_ctx *c, int step0, int nsteps, bool time_kernel, float *ms_k
     }
     for (int n = step0; n < step0 + nsteps; ++n)
     {
-        if (c->nuniq > 0)
+        const tlm::PulseView pv = pulse_view(c, n);
+        tlm::PlanArgs pl = {};
+        if (c->pf != TLM_PULSE_F32)
         {
-            int b = 128, g = (c->nuniq + b - 1) / b;
-            hipLaunchKernelGGL(tlm::k_inject, dim3(g), dim3(b), 0, c->stream, c->A, c->P, c->ucell,
-                               c->ubeg, c->spol, c->samp, c->nuniq, c->sv, n);
+            const int cur = scale_buf(c, n);
+            pl.nz = c->nz;
+            pl.mreg = c->mreg;
+            pl.Q = c->qinj;
+            pl.slab = c->slab;
+            pl.sv = c->sv;
+            pl.n = n;
+            pl.nsteps = c->nsteps;
+            pl.rp = std::max(1.0f, std::fabs(c->rho));
+            pl.fmax = format_max(c->pf);
+            pl.isc_cur = c->iscl[cur];
+            pl.sc_next = c->scl[cur ^ 1];
+            pl.isc_next = c->iscl[cur ^ 1];
+            pl.flag = c->flag;
+        }
+        if (c->nuniq + pl.nz > 0)
+        {
+            int b = 128, g = (c->nuniq + pl.nz + b - 1) / b;
+            hipLaunchKernelGGL(tlm::k_inject, dim3(g), dim3(b), 0, c->stream, pv, c->ucell,
+                               c->ubeg, c->spol, c->samp, c->nuniq, c->sv, n, pl);
         }
         const bool plane_now = c->pa >= 0 && (n % 2) == 0;
         const int pn = plane_now ? c->pn1 * c->pn2 : 0;
@@ -279,8 +241,7 @@ int queue_steps(tlm_ctx *c, int step0, int nsteps, bool time_kernel, float *ms_k
             m.Pp = c->Pp;
             m.code = c->code;
             m.plane = c->plane;
-            m.A = c->A;
-            m.P = c->P;
+            m.pv = pv;
             m.n = n;
             m.settle = c->settle;
             m.cph = c->cph;
@@ -358,13 +319,10 @@ int queue_steps(tlm_ctx *c, int step0, int nsteps, bool time_kernel, float *ms_k
     {
         if (step0 != c->dft_done)
             return fail(c, TLM_ERR_STATE, "probe DFT needs contiguous step blocks from step 0");
-        int tot = c->npr * 3 * c->nfreq, b = 128;
-        hipLaunchKernelGGL(tlm::k_probe_dft, dim3((tot + b - 1) / b), dim3(b), 0, c->stream,
-                           c->prrec, c->npr, c->ctab, c->stab, c->nfreq, step0, step0 + nsteps,
-                           c->dft_re, c->dft_im);
-        c->dft_done = step0 + nsteps;
+        queue_probe_dft(c, step0, step0 + nsteps, c->stream);
     }
     c->steps_run_hi = std::max(c->steps_run_hi, step0 + nsteps);
+    c->next_step = step0 + nsteps;
     if (time_kernel && ms_kernel)
     {
         float kern_ms = 0.f;
@@ -381,8 +339,10 @@ int queue_steps(tlm_ctx *c, int step0, int nsteps, bool time_kernel, float *ms_k
     return TLM_OK;
 }
 
-// Chunked AoS <-> SoA conversion through a bounded staging buffer.
-int convert(tlm_ctx *c, const float *host_in, float *host_out, float *soa, int ncomp)
+// Chunked AoS <-> SoA conversion through a bounded staging buffer: pulses
+// (ncomp 12, any storage format, through pv) or FP32 stubs (ncomp 3, soa).
+int convert(tlm_ctx *c, const float *host_in, float *host_out, const tlm::PulseView *pv,
+            float *soa, int ncomp)
 {
     const long long ncell = (long long)c->nx * c->ny * c->nz;
     const long long chunk = std::min<long long>(ncell, (long long)8 << 20); // 8 M cells
@@ -404,13 +364,21 @@ int convert(tlm_ctx *c, const float *host_in, float *host_out, float *soa, int n
                 dfree(c, stage);
                 return hip_fail(c, e, "hipMemcpyAsync");
             }
-            hipLaunchKernelGGL(tlm::k_aos_to_soa, dim3((unsigned)g), dim3(b), 0, c->stream, stage,
-                               soa, c->P, ncomp, c->nxy, c->Pp, c0, cnt);
+            if (pv)
+                hipLaunchKernelGGL(tlm::k_pulses_in, dim3((unsigned)g), dim3(b), 0, c->stream, stage,
+                                   *pv, c->nxy, c0, cnt);
+            else
+                hipLaunchKernelGGL(tlm::k_aos_to_soa, dim3((unsigned)g), dim3(b), 0, c->stream, stage,
+                                   soa, c->P, ncomp, c->nxy, c->Pp, c0, cnt);
         }
         else
         {
-            hipLaunchKernelGGL(tlm::k_soa_to_aos, dim3((unsigned)g), dim3(b), 0, c->stream, soa,
-                               stage, c->P, ncomp, c->nxy, c->Pp, c0, cnt);
+            if (pv)
+                hipLaunchKernelGGL(tlm::k_pulses_out, dim3((unsigned)g), dim3(b), 0, c->stream, *pv,
+                                   stage, c->nxy, c0, cnt);
+            else
+                hipLaunchKernelGGL(tlm::k_soa_to_aos, dim3((unsigned)g), dim3(b), 0, c->stream, soa,
+                                   stage, c->P, ncomp, c->nxy, c->Pp, c0, cnt);
             hipError_t e = hipMemcpyAsync(host_out + c0 * ncomp, stage, cnt * ncomp * sizeof(float),
                                           hipMemcpyDeviceToHost, c->stream);
             if (e != hipSuccess)
@@ -430,7 +398,106 @@ int convert(tlm_ctx *c, const float *host_in, float *host_out, float *soa, int n
     return TLM_OK;
 }
 
-} // namespace
+// 16-bit formats, first run after tlm_upload: decide which slabs keep FP32
+// (within `halo` slabs of a soft source), lay the slabs out, choose the
+// first scales from the uploaded state and the first injection
+// (E_k = exponent of max|state in slab k| + Q_k*|sv(step0)|), and store the
+// state rounded to those scales.
+int materialize(tlm_ctx *c, int step0)
+{
+    const int nz = c->nz;
+    const long long slab = c->nxy;
+    std::vector<float> Q(nz, 0.0f);
+    std::vector<char> hot(nz, 0);
+    for (size_t s = 0; s < c->src_cells.size(); ++s)
+    {
+        const int k = (int)(c->src_cells[s] / slab);
+        Q[k] = Q[k] + 0.5f * std::fabs(c->src_amp[s]); // list order, as the oracle adds them
+        if (c->halo >= 0)
+            for (int d = -c->halo; d <= c->halo; ++d)
+                if (k + d >= 0 && k + d < nz)
+                    hot[k + d] = 1;
+    }
+    std::vector<uint2> sl(nz);
+    unsigned long long off = 0;
+    c->nhot = 0;
+    for (int k = 0; k < nz; ++k)
+    {
+        // pair planes: two ports per cell, 8 bytes in an FP32 slab, 4 in a 16-bit one
+        sl[k] = make_uint2((unsigned)off, hot[k] ? 0u : 1u);
+        off += (unsigned long long)c->Pp * (hot[k] ? 8u : 4u);
+        c->nhot += hot[k];
+    }
+    if (off > (1ULL << 32) - (1ULL << 20))
+        return fail(c, TLM_ERR_UNSUPPORTED,
+                    "16-bit storage: a pair plane exceeds 32-bit byte offsets (too many cells)");
+    c->PB = (long long)((off + 255) / 256 * 256);
+    // slab maxima of the uploaded state
+    std::vector<float> M(nz, 0.0f);
+    if (!c->pendV.empty())
+        for (int k = 0; k < nz; ++k)
+        {
+            float m = 0.0f;
+            const float *v = c->pendV.data() + (size_t)k * slab * 12;
+            for (long long i = 0; i < slab * 12; ++i)
+                m = std::max(m, std::fabs(v[i]));
+            if (!c->pendVs.empty())
+            {
+                const float *w = c->pendVs.data() + (size_t)k * slab * 3;
+                for (long long i = 0; i < slab * 3; ++i)
+                    m = std::max(m, std::fabs(w[i]));
+            }
+            M[k] = m;
+        }
+    float sv0 = 0.0f;
+    HIPCHK(c, hipMemcpy(&sv0, c->sv + step0, sizeof(float), hipMemcpyDeviceToHost));
+    std::vector<float> sc(nz), isc(nz);
+    for (int k = 0; k < nz; ++k)
+    {
+        const int E = tlm::plan_exponent(M[k] + Q[k] * std::fabs(sv0));
+        sc[k] = tlm::pow2_float(E);
+        isc[k] = tlm::pow2_float(-E);
+    }
+    for (char **buf : {&c->A, &c->B})
+    {
+        dfree(c, *buf);
+        int rc = dalloc(c, buf, (size_t)c->PB * 6);
+        if (rc)
+            return rc;
+    }
+    int rc;
+    if ((rc = upload(c, &c->slab, sl.data(), sl.size())) ||
+        (rc = upload(c, &c->scl[0], sc.data(), sc.size())) ||
+        (rc = upload(c, &c->iscl[0], isc.data(), isc.size())) ||
+        (rc = upload(c, &c->scl[1], sc.data(), sc.size())) ||
+        (rc = upload(c, &c->iscl[1], isc.data(), isc.size())) ||
+        (rc = upload(c, &c->mreg, M.data(), M.size())) ||
+        (rc = upload(c, &c->qinj, Q.data(), Q.size())) ||
+        (rc = upload<float>(c, &c->mpart, nullptr, (size_t)c->strips * nz)) ||
+        (rc = upload<int>(c, &c->flag, nullptr, 1)))
+        return rc;
+    HIPCHK(c, hipMemsetAsync(c->flag, 0, sizeof(int), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->A, 0, (size_t)c->PB * 6, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->B, 0, (size_t)c->PB * 6, c->stream));
+    c->base = step0;
+    c->materialized = true;
+    if (!c->pendV.empty())
+    {
+        const tlm::PulseView pv = pulse_view(c, step0);
+        if ((rc = convert(c, c->pendV.data(), nullptr, &pv, nullptr, 12)))
+            return rc;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->pendV.clear();
+    c->pendV.shrink_to_fit();
+    c->pendVs.clear();
+    c->pendVs.shrink_to_fit();
+    return TLM_OK;
+}
+
+} // namespace tlmi
+
+using namespace tlmi;
 
 extern "C" {
 
@@ -462,8 +529,9 @@ int tlm_create(const tlm_grid_desc *d, tlm_ctx **out)
         return fail(nullptr, TLM_ERR_ARG, "tlm_create: desc is NULL");
     if (d->nx <= 0 || d->ny <= 0 || d->nz <= 0)
         return fail(nullptr, TLM_ERR_ARG, "empty grid");
-    if (d->pulse_type != TLM_PULSE_F32)
-        return fail(nullptr, TLM_ERR_UNSUPPORTED, "only FP32 pulse storage in this entry point");
+    if (d->pulse_type != TLM_PULSE_F32 && d->pulse_type != TLM_PULSE_F16 &&
+        d->pulse_type != TLM_PULSE_BF16)
+        return fail(nullptr, TLM_ERR_ARG, "unknown pulse storage type");
     {
         // one pulse plane must be addressable with 32-bit byte offsets
         const long long pp = ((long long)d->nx * d->ny + 63) / 64 * 64;
@@ -489,6 +557,9 @@ int tlm_create(const tlm_grid_desc *d, tlm_ctx **out)
     c->P = c->Pp * c->nz;
     c->dl = d->dl;
     c->rho = d->rho;
+    c->pf = d->pulse_type;
+    c->halo = d->fp32_halo < 0 ? -1 : (d->fp32_halo == 0 ? 8 : d->fp32_halo);
+    c->PB = c->P * 4;
     c->inv_dl = -1.0f / d->dl;
     choose_shape(c);
     auto bail = [&](int rc) {
@@ -506,8 +577,10 @@ int tlm_create(const tlm_grid_desc *d, tlm_ctx **out)
             return bail(hip_fail(c, e, "hipEventCreate"));
     int rc;
     c->epart_cap = energy_parts(c);
-    if ((rc = dalloc(c, &c->A, (size_t)c->P * 12)) || (rc = dalloc(c, &c->B, (size_t)c->P * 12)) ||
-        (rc = dalloc(c, &c->code, (size_t)c->P)) || (rc = dalloc(c, &c->epart, c->epart_cap)) ||
+    if (c->pf == TLM_PULSE_F32 &&
+        ((rc = dalloc(c, &c->A, (size_t)c->PB * 12)) || (rc = dalloc(c, &c->B, (size_t)c->PB * 12))))
+        return bail(rc);
+    if ((rc = dalloc(c, &c->code, (size_t)c->P)) || (rc = dalloc(c, &c->epart, c->epart_cap)) ||
         (rc = dalloc(c, &c->epart2, 1024)))
         return bail(rc);
     *out = c;
@@ -518,6 +591,8 @@ void tlm_destroy(tlm_ctx *c)
 {
     if (!c)
         return;
+    if (c->batch)
+        (void)tlm_batch_remove(c->batch, c);
     (void)hipSetDevice(c->dev);
     if (c->stream)
         (void)hipStreamSynchronize(c->stream);
@@ -542,6 +617,8 @@ const char *tlm_last_error(const tlm_ctx *c)
 int tlm_upload(tlm_ctx *c, const uint8_t *mat, const float *yhat, const float *ghat, int nmat,
                const float *V, const float *Vs)
 {
+    if (c && c->batch)
+        return fail(c, TLM_ERR_STATE, "context is a batch member");
     if (!c || !mat || nmat < 0 || (nmat > 0 && (!yhat || !ghat)))
         return fail(c, TLM_ERR_ARG, "tlm_upload: bad arguments");
     HIPCHK(c, hipSetDevice(c->dev));
@@ -599,8 +676,11 @@ int tlm_upload(tlm_ctx *c, const uint8_t *mat, const float *yhat, const float *g
         return rc;
     c->nmat = nmat;
     c->stubs = any_diel;
-    HIPCHK(c, hipMemsetAsync(c->A, 0, (size_t)c->P * 12 * sizeof(float), c->stream));
-    HIPCHK(c, hipMemsetAsync(c->B, 0, (size_t)c->P * 12 * sizeof(float), c->stream));
+    if (c->pf == TLM_PULSE_F32)
+    {
+        HIPCHK(c, hipMemsetAsync(c->A, 0, (size_t)c->PB * 12, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->B, 0, (size_t)c->PB * 12, c->stream));
+    }
     dfree(c, c->Vs);
     if (c->stubs)
     {
@@ -609,9 +689,27 @@ int tlm_upload(tlm_ctx *c, const uint8_t *mat, const float *yhat, const float *g
         HIPCHK(c, hipMemsetAsync(c->Vs, 0, (size_t)c->P * 3 * sizeof(float), c->stream));
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (V && (rc = convert(c, V, nullptr, c->A, 12)))
-        return rc;
-    if (Vs && c->stubs && (rc = convert(c, Vs, nullptr, c->Vs, 3)))
+    if (c->pf == TLM_PULSE_F32)
+    {
+        const tlm::PulseView pv = pulse_view(c, 0);
+        if (V && (rc = convert(c, V, nullptr, &pv, nullptr, 12)))
+            return rc;
+    }
+    else
+    {
+        // the 16-bit scales depend on the sources and the first step: the
+        // state is stored at the first tlm_run
+        c->materialized = false;
+        c->pendV.clear();
+        c->pendVs.clear();
+        if (V)
+        {
+            c->pendV.assign(V, V + (size_t)ncell * 12);
+            if (Vs && c->stubs)
+                c->pendVs.assign(Vs, Vs + (size_t)ncell * 3);
+        }
+    }
+    if (Vs && c->stubs && (rc = convert(c, Vs, nullptr, nullptr, c->Vs, 3)))
         return rc;
     c->uploaded = true;
     c->energy_steps.clear();
@@ -622,8 +720,13 @@ int tlm_upload(tlm_ctx *c, const uint8_t *mat, const float *yhat, const float *g
 
 int tlm_set_sources(tlm_ctx *c, const int64_t *cells, const int32_t *pol, const float *amp, int n)
 {
+    if (c && c->batch)
+        return fail(c, TLM_ERR_STATE, "context is a batch member");
     if (!c || n < 0 || (n > 0 && (!cells || !pol || !amp)))
         return fail(c, TLM_ERR_ARG, "tlm_set_sources: bad arguments");
+    if (c->pf != TLM_PULSE_F32 && c->materialized)
+        return fail(c, TLM_ERR_STATE,
+                    "16-bit pulse storage: sources are fixed once the first step has run");
     // group by cell, keeping list order within a cell
     std::map<long long, std::vector<int>> by;
     for (int s = 0; s < n; ++s)
@@ -646,6 +749,8 @@ int tlm_set_sources(tlm_ctx *c, const int64_t *cells, const int32_t *pol, const 
         }
     }
     ub.push_back((int)sp.size());
+    c->src_cells.assign(cells, cells + n);
+    c->src_amp.assign(amp, amp + n);
     int rc;
     if ((rc = upload(c, &c->ucell, uc.data(), uc.size())) ||
         (rc = upload(c, &c->ubeg, ub.data(), ub.size())) ||
@@ -659,6 +764,8 @@ int tlm_set_sources(tlm_ctx *c, const int64_t *cells, const int32_t *pol, const 
 int tlm_set_schedule(tlm_ctx *c, const float *sv, const float *cph, const float *sph, int nsteps,
                      int settle)
 {
+    if (c && c->batch)
+        return fail(c, TLM_ERR_STATE, "context is a batch member");
     if (!c || nsteps <= 0 || !sv || !cph || !sph)
         return fail(c, TLM_ERR_ARG, "tlm_set_schedule: bad arguments");
     int rc;
@@ -710,6 +817,8 @@ int tlm_set_energy_every(tlm_ctx *c, int every)
 
 int tlm_set_plane(tlm_ctx *c, int axis, int index)
 {
+    if (c && c->batch)
+        return fail(c, TLM_ERR_STATE, "context is a batch member");
     if (!c || axis > 2)
         return fail(c, TLM_ERR_ARG, "tlm_set_plane: bad arguments");
     dfree(c, c->plane);
@@ -731,6 +840,8 @@ int tlm_set_plane(tlm_ctx *c, int axis, int index)
 
 int tlm_set_frames(tlm_ctx *c, int every, int max_frames)
 {
+    if (c && c->batch)
+        return fail(c, TLM_ERR_STATE, "context is a batch member");
     if (!c || every < 0 || max_frames < 0)
         return fail(c, TLM_ERR_ARG, "tlm_set_frames: bad arguments");
     dfree(c, c->frame_js);
@@ -793,6 +904,8 @@ int tlm_read_frames(tlm_ctx *c, int32_t *steps, float *js, float *planes, int ca
 
 int tlm_set_probes(tlm_ctx *c, const int64_t *cells, int n, float inv_dl)
 {
+    if (c && c->batch)
+        return fail(c, TLM_ERR_STATE, "context is a batch member");
     if (!c || n < 0 || (n > 0 && !cells))
         return fail(c, TLM_ERR_ARG, "tlm_set_probes: bad arguments");
     if (c->nsteps <= 0)
@@ -819,6 +932,8 @@ int tlm_set_probes(tlm_ctx *c, const int64_t *cells, int n, float inv_dl)
 
 int tlm_set_probe_dft(tlm_ctx *c, const double *ctab, const double *stab, int nfreq)
 {
+    if (c && c->batch)
+        return fail(c, TLM_ERR_STATE, "context is a batch member");
     if (!c || nfreq < 0 || (nfreq > 0 && (!ctab || !stab)))
         return fail(c, TLM_ERR_ARG, "tlm_set_probe_dft: bad arguments");
     if (c->nsteps <= 0)
@@ -841,6 +956,8 @@ int tlm_set_probe_dft(tlm_ctx *c, const double *ctab, const double *stab, int nf
 
 int tlm_set_ports(tlm_ctx *c, const int64_t *cells, const int32_t *off, const int32_t *pol, int np)
 {
+    if (c && c->batch)
+        return fail(c, TLM_ERR_STATE, "context is a batch member");
     if (!c || np < 0 || (np > 0 && (!off || !pol)))
         return fail(c, TLM_ERR_ARG, "tlm_set_ports: bad arguments");
     if (c->nsteps <= 0)
@@ -893,6 +1010,8 @@ int tlm_set_ports(tlm_ctx *c, const int64_t *cells, const int32_t *off, const in
 
 int tlm_set_faces(tlm_ctx *c, const int64_t *cells, const int32_t *axes, int n)
 {
+    if (c && c->batch)
+        return fail(c, TLM_ERR_STATE, "context is a batch member");
     if (!c || n < 0 || (n > 0 && (!cells || !axes)))
         return fail(c, TLM_ERR_ARG, "tlm_set_faces: bad arguments");
     std::vector<long long> fc(n);
@@ -921,6 +1040,8 @@ int tlm_set_faces(tlm_ctx *c, const int64_t *cells, const int32_t *axes, int n)
 
 int tlm_set_huygens(tlm_ctx *c, const int64_t *cells, const int32_t *axes, int n)
 {
+    if (c && c->batch)
+        return fail(c, TLM_ERR_STATE, "context is a batch member");
     if (!c || n < 0 || (n > 0 && (!cells || !axes)))
         return fail(c, TLM_ERR_ARG, "tlm_set_huygens: bad arguments");
     std::vector<long long> hc(n);
@@ -958,7 +1079,104 @@ int tlm_sync(tlm_ctx *c)
     HIPCHK(c, hipSetDevice(c->dev));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipGetLastError());
+    if (c->flag)
+    {
+        int f = 0;
+        HIPCHK(c, hipMemcpy(&f, c->flag, sizeof(int), hipMemcpyDeviceToHost));
+        if (f)
+            return fail(c, TLM_ERR_RANGE,
+                        "16-bit pulse storage: a value exceeded the range its slab scale allows");
+    }
     return TLM_OK;
+}
+
+int tlm_read_scales(tlm_ctx *c, float *scale, float *smax, int32_t *fmt)
+{
+    if (!c)
+        return TLM_ERR_ARG;
+    if (c->pf == TLM_PULSE_F32 || !c->materialized)
+        return 0;
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (scale)
+        HIPCHK(c, hipMemcpy(scale, c->scl[scale_buf(c, c->next_step)], c->nz * sizeof(float),
+                            hipMemcpyDeviceToHost));
+    if (smax)
+        HIPCHK(c, hipMemcpy(smax, c->mreg, c->nz * sizeof(float), hipMemcpyDeviceToHost));
+    if (fmt)
+    {
+        std::vector<uint2> sl(c->nz);
+        HIPCHK(c, hipMemcpy(sl.data(), c->slab, c->nz * sizeof(uint2), hipMemcpyDeviceToHost));
+        for (int k = 0; k < c->nz; ++k)
+            fmt[k] = sl[k].y ? c->pf : TLM_PULSE_F32;
+    }
+    return c->nz;
+}
+
+int tlm_device_memory(int device, size_t *free_bytes, size_t *total_bytes)
+{
+    if (device < 0 || device >= tlm_device_count())
+        return TLM_ERR_NO_DEVICE;
+    if (hipSetDevice(device) != hipSuccess)
+        return TLM_ERR_HIP;
+    size_t f = 0, t = 0;
+    if (hipMemGetInfo(&f, &t) != hipSuccess)
+        return TLM_ERR_HIP;
+    if (free_bytes)
+        *free_bytes = f;
+    if (total_bytes)
+        *total_bytes = t;
+    return TLM_OK;
+}
+
+int64_t tlm_estimate_bytes(const tlm_grid_desc *d, int has_stubs)
+{
+    if (!d || d->nx <= 0 || d->ny <= 0 || d->nz <= 0)
+        return TLM_ERR_ARG;
+    const long long Pp = ((long long)d->nx * d->ny + 63) / 64 * 64, P = Pp * d->nz;
+    const long long sp = d->pulse_type == TLM_PULSE_F32 ? 4 : 2; // FP32 halo slabs not counted
+    long long bytes = 2 * 12 * P * sp + 2 * P;                   // pulses A, B and cell codes
+    if (has_stubs)
+        bytes += 3 * P * 4;
+    bytes += (P / 64) * 8 + (1 << 20); // energy / slab-maximum partials, small tables
+    return bytes;
+}
+
+int tlm_device_encode(int device, int fmt, const float *in, float inv_scale, uint16_t *out,
+                      int64_t n)
+{
+    if ((fmt != TLM_PULSE_F16 && fmt != TLM_PULSE_BF16) || n < 0 || (n && (!in || !out)))
+        return TLM_ERR_ARG;
+    if (device < 0 || device >= tlm_device_count())
+        return TLM_ERR_NO_DEVICE;
+    if (hipSetDevice(device) != hipSuccess)
+        return TLM_ERR_HIP;
+    if (n == 0)
+        return TLM_OK;
+    float *din = nullptr;
+    unsigned short *dout = nullptr;
+    if (hipMalloc((void **)&din, n * sizeof(float)) != hipSuccess)
+        return TLM_ERR_NOMEM;
+    if (hipMalloc((void **)&dout, n * sizeof(unsigned short)) != hipSuccess)
+    {
+        (void)hipFree(din);
+        return TLM_ERR_NOMEM;
+    }
+    int rc = TLM_OK;
+    if (hipMemcpy(din, in, n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+        rc = TLM_ERR_HIP;
+    else
+    {
+        const int b = 256;
+        hipLaunchKernelGGL(tlm::k_encode, dim3((unsigned)((n + b - 1) / b)), dim3(b), 0, 0, fmt, din,
+                           inv_scale, dout, (long long)n);
+        if (hipDeviceSynchronize() != hipSuccess ||
+            hipMemcpy(out, dout, n * sizeof(unsigned short), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = TLM_ERR_HIP;
+    }
+    (void)hipFree(din);
+    (void)hipFree(dout);
+    return rc;
 }
 
 int tlm_time_steps(tlm_ctx *c, int step0, int nsteps, float *ms_total, float *ms_kernel)
@@ -996,6 +1214,8 @@ int tlm_launch_shape(const tlm_ctx *c, int *rows, int *planes, int *groups, int 
 
 int tlm_set_tuning(tlm_ctx *c, const tlm_tuning *t)
 {
+    if (c && c->batch)
+        return fail(c, TLM_ERR_STATE, "context is a batch member");
     if (!c || !t || t->kernel != 0 || t->planes < 0 || t->block_waves < 0 ||
         t->block_waves > 4 || t->nontemporal < 0 || t->nontemporal > 1 || t->grid_blocks < 0)
         return fail(c, TLM_ERR_ARG, "tlm_set_tuning: bad arguments");
@@ -1028,13 +1248,26 @@ int tlm_read_state(tlm_ctx *c, float *V, float *Vs)
         return fail(c, TLM_ERR_ARG, "tlm_read_state: bad arguments");
     HIPCHK(c, hipSetDevice(c->dev));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    int rc = convert(c, nullptr, V, c->A, 12);
-    if (rc)
-        return rc;
+    const size_t ncell = (size_t)c->nx * c->ny * c->nz;
+    if (c->pf != TLM_PULSE_F32 && !c->materialized)
+    {
+        // not run yet: the uploaded state
+        if (c->pendV.empty())
+            std::memset(V, 0, sizeof(float) * 12 * ncell);
+        else
+            std::memcpy(V, c->pendV.data(), sizeof(float) * 12 * ncell);
+    }
+    else
+    {
+        const tlm::PulseView pv = pulse_view(c, c->next_step);
+        int rc = convert(c, nullptr, V, &pv, nullptr, 12);
+        if (rc)
+            return rc;
+    }
     if (Vs)
     {
         if (c->stubs)
-            return convert(c, nullptr, Vs, c->Vs, 3);
+            return convert(c, nullptr, Vs, nullptr, c->Vs, 3);
         std::memset(Vs, 0, sizeof(float) * 3 * (size_t)c->nx * c->ny * c->nz);
     }
     return TLM_OK;

@@ -26,6 +26,7 @@
 #include "scn_node.h"
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 namespace tlm {
 
@@ -33,18 +34,50 @@ constexpr float ETA0 = 376.730313f; // TlmSolver.cpp:16
 constexpr int kWave = 64;
 constexpr int kNumXcd = 8;          // MI355X: 8 accelerator complex dies, one L2 each
 
+// Pulse storage formats. FP32 keeps every slab in floats; FP16 / BF16 store
+// each z-slab either in floats (slabs near soft sources) or as 16-bit codes
+// times a per-slab power of two 2^E (decode: code * 2^E, encode: round to
+// nearest even of v * 2^-E). The scatter arithmetic is FP32 throughout.
+enum : int
+{
+    PF_F32 = 0,
+    PF_F16 = 1,
+    PF_BF16 = 2
+};
+
+template <int PF> __device__ __forceinline__ float dec16(unsigned h)
+{
+    if (PF == PF_F16)
+        return (float)__builtin_bit_cast(_Float16, (unsigned short)h);
+    return __uint_as_float(h << 16);
+}
+template <int PF> __device__ __forceinline__ unsigned short enc16(float v)
+{
+    if (PF == PF_F16)
+        return __builtin_bit_cast(unsigned short, (_Float16)v); // v_cvt_f16_f32, round to nearest even
+    unsigned u = __float_as_uint(v);
+    u += 0x7fffu + ((u >> 16) & 1u); // bfloat16 round to nearest even
+    return (unsigned short)(u >> 16);
+}
+
 struct StepArgs
 {
-    const float *A;
-    float *B;
-    float *Vs;
+    const char *A;        // incident pulses of step n: 12 planes of PB bytes
+    char *B;              // incident pulses of step n+1
+    float *Vs;            // stubs: 3 planes of P floats
     const uint16_t *code; // cell code (material + neighbour-PEC bits)
     const float4 *props;  // per dielectric id: (yh, gh, den, 0)
     double *epart;        // energy partial per wave task
+    // 16-bit formats only
+    const uint2 *slab;    // per slab: (byte offset inside a plane, 1 if 16-bit storage)
+    const float *dsc;     // per slab: 2^E_n (decode A)
+    const float *esc;     // per slab: 2^-E_{n+1} (encode B)
+    float *mpart;         // per (slab, strip): largest |value| the wave's cells send
     int nx, ny, nz;
     int nxy;             // nx*ny
-    long long Pp;        // slab pitch: roundup(nx*ny, 64)
-    long long P;         // floats per pulse plane = Pp*nz
+    long long Pp;        // slab pitch: roundup(nx*ny, 64) cells
+    long long P;         // cells per plane = Pp*nz (stub planes)
+    long long PB;        // bytes per pulse plane
     int strips;          // 64-cell strips per slab: Pp/64
     int zc, nzc;         // planes per task, number of z chunks
     int tasks;           // strips * nzc
@@ -54,13 +87,13 @@ struct StepArgs
 
 // Loads and stores through a uniform plane base (SGPR pair) and a 32-bit
 // byte offset per lane (one VGPR), the global_load saddr+voffset form.
-__device__ __forceinline__ float ldo(const float *base, unsigned boff)
+__device__ __forceinline__ float ldo(const char *base, unsigned boff)
 {
-    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + boff);
+    return *reinterpret_cast<const float *>(base + boff);
 }
-template <bool NT> __device__ __forceinline__ void sto(float *base, unsigned boff, float v)
+template <bool NT> __device__ __forceinline__ void sto(char *base, unsigned boff, float v)
 {
-    float *p = reinterpret_cast<float *>(reinterpret_cast<char *>(base) + boff);
+    float *p = reinterpret_cast<float *>(base + boff);
     if (NT)
         __builtin_nontemporal_store(v, p);
     else
@@ -79,6 +112,54 @@ enum : unsigned
     NB_ZM = 1u << 12,
     NB_ZP = 1u << 13
 };
+
+// One slab as seen by one lane: byte offset of the lane's cell inside a
+// plane, element size, format and scales (wave-uniform apart from bo).
+struct SlabRef
+{
+    unsigned bo, es;
+    bool h16;
+    float ds, isc;
+};
+template <int PF> __device__ __forceinline__ SlabRef slab_ref(const StepArgs &a, int k, unsigned mo, int f)
+{
+    if (PF == PF_F32)
+        return {mo * 4u, 4u, false, 1.0f, 1.0f};
+    const uint2 s = a.slab[k];
+    // the slab's format is the same for the whole wave: a scalar branch
+    const bool h = __builtin_amdgcn_readfirstlane(s.y) != 0u;
+    const unsigned es = h ? 2u : 4u;
+    return {s.x + (unsigned)f * es, es, h, a.dsc[k], a.esc[k]};
+}
+template <int PF> __device__ __forceinline__ float ldp(const char *base, unsigned boff, const SlabRef &s)
+{
+    if (PF != PF_F32 && s.h16)
+        return dec16<PF>(*reinterpret_cast<const unsigned short *>(base + boff)) * s.ds;
+    return ldo(base, boff);
+}
+// Same with the format known at compile time (the slab's own accesses).
+template <int PF, bool H16> __device__ __forceinline__ float ldh(const char *base, unsigned boff, const SlabRef &s)
+{
+    if (H16)
+        return dec16<PF>(*reinterpret_cast<const unsigned short *>(base + boff)) * s.ds;
+    return ldo(base, boff);
+}
+template <int PF, bool NT, bool H16>
+__device__ __forceinline__ void sth(char *base, unsigned boff, const SlabRef &s, float v)
+{
+    if (H16)
+        *reinterpret_cast<unsigned short *>(base + boff) = enc16<PF>(v * s.isc);
+    else
+        sto<NT>(base, boff, v);
+}
+template <int PF, bool NT>
+__device__ __forceinline__ void stp(char *base, unsigned boff, const SlabRef &s, float v)
+{
+    if (PF != PF_F32 && s.h16)
+        *reinterpret_cast<unsigned short *>(base + boff) = enc16<PF>(v * s.isc);
+    else
+        sto<NT>(base, boff, v);
+}
 
 // Workgroups are dispatched to the 8 XCDs round robin. Each XCD gets one
 // contiguous range of task groups, walked by its own workgroups in a
@@ -117,8 +198,13 @@ __device__ __forceinline__ XcdRange xcd_range(int b, int G, int ngroups)
 // The owner writes its own slot instead whenever the link is broken: own
 // cell PEC (value kept, times rho on an outer face), outer face (rho * out),
 // neighbour PEC (-out). This is connect(), TlmSolver.cpp:563-583.
+//
+// 16-bit formats: a value is rounded once, where it is stored, with the
+// scale of the slab it is stored in; each wave also records the largest
+// |value| its cells send (|out| and new stubs of non-PEC cells, |in| of PEC
+// cells), which bounds the scales of the next steps.
 // One wave task: a 64-cell strip of slabs [k0, k1).
-template <bool STUBS, bool ENERGY, bool NT>
+template <bool STUBS, bool ENERGY, bool NT, int PF>
 __device__ __forceinline__ void strip_task(const StepArgs &a, int task, int lane, double &esum)
 {
     const int tz = task / a.strips, t = task - tz * a.strips;
@@ -127,15 +213,13 @@ __device__ __forceinline__ void strip_task(const StepArgs &a, int task, int lane
     const int j = f / a.nx, i = f - j * a.nx;
     const int k0 = tz * a.zc, k1 = min(a.nz, k0 + a.zc);
     const float rho = a.rho;
-    // byte strides (all offsets of one plane fit in 32 bits)
-    const unsigned bz = (unsigned)a.Pp * 4u, by = (unsigned)a.nx * 4u;
-    const float *A[12];
-    float *B[12];
+    const char *A[12];
+    char *B[12];
 #pragma unroll
     for (int p = 0; p < 12; ++p)
     {
-        A[p] = a.A + p * a.P;
-        B[p] = a.B + p * a.P;
+        A[p] = a.A + p * a.PB;
+        B[p] = a.B + p * a.PB;
     }
     // x neighbour inside the wave; otherwise (strip seam) the two cells push
     const bool hasL = act && lane > 0 && i > 0;
@@ -145,177 +229,484 @@ __device__ __forceinline__ void strip_task(const StepArgs &a, int task, int lane
     float zpo0 = 0.f, zpo1 = 0.f; // previous slab: reflected z+ pulses
     bool pecPrev = false;
     unsigned mo = (unsigned)(k0 * a.Pp + f); // cell index in the slab-padded space
-    unsigned bo = mo * 4u;                   // its byte offset inside a plane
+    SlabRef sp = {0u, 4u, false, 1.0f, 1.0f}; // previous slab
 
-    for (int k = k0; k < k1; ++k, mo += (unsigned)a.Pp, bo += bz)
+    for (int k = k0; k < k1; ++k, mo += (unsigned)a.Pp)
     {
-        float o[12];
-        bool pec = true;
-        unsigned code = 0;
-        if (act)
-        {
-            code = ldc(a.code, mo);
-            const unsigned m = code & 0xFFu;
-            pec = (m == 1u);
-            float v[12];
-#pragma unroll
-            for (int p = 0; p < 12; ++p)
-                v[p] = ldo(A[p], bo);
-            if (ENERGY)
-            {
-#pragma unroll
-                for (int p = 0; p < 12; ++p)
-                    esum += (double)v[p] * (double)v[p];
-            }
-            if (STUBS && m >= 2u)
-            {
-                float *S0 = a.Vs, *S1 = a.Vs + a.P, *S2 = a.Vs + 2 * a.P;
-                float st[3] = {ldo(S0, bo), ldo(S1, bo), ldo(S2, bo)};
-                const float4 pr = a.props[m - 2u];
-                scn::scatter_diel(v, st, pr.x, pr.z, o);
-                sto<false>(S0, bo, st[0]);
-                sto<false>(S1, bo, st[1]);
-                sto<false>(S2, bo, st[2]);
-            }
-            else
-                scn::scatter_air(v, o);
-        }
-        else
-        {
-#pragma unroll
-            for (int p = 0; p < 12; ++p)
-                o[p] = 0.f;
-        }
-        // every lane takes part in the shuffles
-        const float L0 = __shfl_up(o[2], 1, kWave), L1 = __shfl_up(o[3], 1, kWave);
-        const float R0 = __shfl_down(o[0], 1, kWave), R1 = __shfl_down(o[1], 1, kWave);
-        if (act)
-        {
-            // ---- x faces: ports 0,1 (x-) and 2,3 (x+) ----
-            {
-                const bool pecL = (code & NB_XM) != 0u, pecR = (code & NB_XP) != 0u;
-#pragma unroll
-                for (int pol = 0; pol < 2; ++pol)
+        const SlabRef s = slab_ref<PF>(a, k, mo, f);
+        // the slab's own loads and stores use one storage format, chosen by a
+        // wave-uniform branch around the whole slab
+        auto slab_body = [&](auto h16_tag) {
+            constexpr bool H16 = decltype(h16_tag)::value;
+                const unsigned bo = s.bo;
+                const unsigned by = (unsigned)a.nx * s.es;
+                float o[12];
+                float mc = 0.f; // 16-bit formats: largest |value| this cell sends
+                bool pec = true;
+                unsigned code = 0;
+                if (act)
                 {
-                    const float oo = o[0 + pol];
-                    if (pec)
+                    code = ldc(a.code, mo);
+                    const unsigned m = code & 0xFFu;
+                    pec = (m == 1u);
+                    float v[12];
+        #pragma unroll
+                    for (int p = 0; p < 12; ++p)
+                        v[p] = ldh<PF, H16>(A[p], bo, s);
+                    if (ENERGY)
                     {
-                        const float old = ldo(A[0 + pol], bo);
-                        sto<NT>(B[0 + pol], bo, bx0 ? rho * old : old);
+        #pragma unroll
+                        for (int p = 0; p < 12; ++p)
+                            esum += (double)v[p] * (double)v[p];
                     }
-                    else if (bx0)
-                        sto<NT>(B[0 + pol], bo, rho * oo);
-                    else if (pecL)
-                        sto<NT>(B[0 + pol], bo, -oo);
-                    else if (hasL)
-                        sto<NT>(B[0 + pol], bo, pol ? L1 : L0);
-                    else // seam with a live neighbour: it fills my slot, I fill its
-                        sto<NT>(B[2 + pol], bo - 4u, oo);
-
-                    const float op = o[2 + pol];
-                    if (pec)
+                    if (STUBS && m >= 2u)
                     {
-                        const float old = ldo(A[2 + pol], bo);
-                        sto<NT>(B[2 + pol], bo, bx1 ? rho * old : old);
-                    }
-                    else if (bx1)
-                        sto<NT>(B[2 + pol], bo, rho * op);
-                    else if (pecR)
-                        sto<NT>(B[2 + pol], bo, -op);
-                    else if (hasR)
-                        sto<NT>(B[2 + pol], bo, pol ? R1 : R0);
-                    else
-                        sto<NT>(B[0 + pol], bo + 4u, op);
-                }
-            }
-            // ---- y faces: ports 4,5 (y-) and 6,7 (y+), sender push ----
-            {
-                const bool pecLo = (code & NB_YM) != 0u, pecHi = (code & NB_YP) != 0u;
-#pragma unroll
-                for (int pol = 0; pol < 2; ++pol)
-                {
-                    const float ol = o[4 + pol], oh = o[6 + pol];
-                    if (pec)
-                    {
-                        const float vl = ldo(A[4 + pol], bo), vh = ldo(A[6 + pol], bo);
-                        sto<NT>(B[4 + pol], bo, by0 ? rho * vl : vl);
-                        sto<NT>(B[6 + pol], bo, by1 ? rho * vh : vh);
+                        float *S0 = a.Vs, *S1 = a.Vs + a.P, *S2 = a.Vs + 2 * a.P;
+                        const unsigned bs = mo * 4u;
+                        float st[3] = {ldo((const char *)S0, bs), ldo((const char *)S1, bs),
+                                       ldo((const char *)S2, bs)};
+                        const float4 pr = a.props[m - 2u];
+                        scn::scatter_diel(v, st, pr.x, pr.z, o);
+                        sto<false>((char *)S0, bs, st[0]);
+                        sto<false>((char *)S1, bs, st[1]);
+                        sto<false>((char *)S2, bs, st[2]);
+                        if (PF != PF_F32)
+                            mc = fmaxf(fmaxf(fabsf(st[0]), fabsf(st[1])), fabsf(st[2]));
                     }
                     else
+                        scn::scatter_air(v, o);
+                    if (PF != PF_F32)
                     {
-                        if (by0)
-                            sto<NT>(B[4 + pol], bo, rho * ol);
-                        else if (pecLo)
-                            sto<NT>(B[4 + pol], bo, -ol);
-                        else
-                            sto<NT>(B[6 + pol], bo - by, ol);
-                        if (by1)
-                            sto<NT>(B[6 + pol], bo, rho * oh);
-                        else if (pecHi)
-                            sto<NT>(B[6 + pol], bo, -oh);
-                        else
-                            sto<NT>(B[4 + pol], bo + by, oh);
+        #pragma unroll
+                        for (int p = 0; p < 12; ++p)
+                            mc = fmaxf(mc, fabsf(pec ? v[p] : o[p]));
                     }
-                }
-            }
-            // ---- z faces: ports 8,9 (z-) and 10,11 (z+) ----
-#pragma unroll
-            for (int pol = 0; pol < 2; ++pol)
-            {
-                const float ol = o[8 + pol];
-                if (k > k0)
-                {
-                    // pull: slab k-1 is in registers
-                    const float zp = pol ? zpo1 : zpo0;
-                    if (pec)
-                        sto<NT>(B[8 + pol], bo, ldo(A[8 + pol], bo));
-                    else
-                        sto<NT>(B[8 + pol], bo, pecPrev ? -ol : zp);
-                    // slab k-1's z+ slot
-                    if (pecPrev)
-                        sto<NT>(B[10 + pol], bo - bz, ldo(A[10 + pol], bo - bz));
-                    else
-                        sto<NT>(B[10 + pol], bo - bz, pec ? -zp : ol);
-                }
-                else if (k == 0)
-                {
-                    sto<NT>(B[8 + pol], bo, rho * (pec ? ldo(A[8 + pol], bo) : ol));
                 }
                 else
                 {
-                    const bool pecLo = (code & NB_ZM) != 0u;
-                    if (pec)
-                        sto<NT>(B[8 + pol], bo, ldo(A[8 + pol], bo));
-                    else if (pecLo)
-                        sto<NT>(B[8 + pol], bo, -ol);
-                    else
-                        sto<NT>(B[10 + pol], bo - bz, ol);
+        #pragma unroll
+                    for (int p = 0; p < 12; ++p)
+                        o[p] = 0.f;
                 }
-                if (k == k1 - 1)
+                // every lane takes part in the shuffles
+                const float L0 = __shfl_up(o[2], 1, kWave), L1 = __shfl_up(o[3], 1, kWave);
+                const float R0 = __shfl_down(o[0], 1, kWave), R1 = __shfl_down(o[1], 1, kWave);
+                if (act)
                 {
-                    const float oh = o[10 + pol];
-                    if (k == a.nz - 1)
-                        sto<NT>(B[10 + pol], bo, rho * (pec ? ldo(A[10 + pol], bo) : oh));
-                    else
+                    // ---- x faces: ports 0,1 (x-) and 2,3 (x+) ----
                     {
-                        const bool pecHi = (code & NB_ZP) != 0u;
-                        if (pec)
-                            sto<NT>(B[10 + pol], bo, ldo(A[10 + pol], bo));
-                        else if (pecHi)
-                            sto<NT>(B[10 + pol], bo, -oh);
+                        const bool pecL = (code & NB_XM) != 0u, pecR = (code & NB_XP) != 0u;
+        #pragma unroll
+                        for (int pol = 0; pol < 2; ++pol)
+                        {
+                            const float oo = o[0 + pol];
+                            if (pec)
+                            {
+                                const float old = ldh<PF, H16>(A[0 + pol], bo, s);
+                                sth<PF, NT, H16>(B[0 + pol], bo, s, bx0 ? rho * old : old);
+                            }
+                            else if (bx0)
+                                sth<PF, NT, H16>(B[0 + pol], bo, s, rho * oo);
+                            else if (pecL)
+                                sth<PF, NT, H16>(B[0 + pol], bo, s, -oo);
+                            else if (hasL)
+                                sth<PF, NT, H16>(B[0 + pol], bo, s, pol ? L1 : L0);
+                            else // seam with a live neighbour: it fills my slot, I fill its
+                                sth<PF, NT, H16>(B[2 + pol], bo - s.es, s, oo);
+
+                            const float op = o[2 + pol];
+                            if (pec)
+                            {
+                                const float old = ldh<PF, H16>(A[2 + pol], bo, s);
+                                sth<PF, NT, H16>(B[2 + pol], bo, s, bx1 ? rho * old : old);
+                            }
+                            else if (bx1)
+                                sth<PF, NT, H16>(B[2 + pol], bo, s, rho * op);
+                            else if (pecR)
+                                sth<PF, NT, H16>(B[2 + pol], bo, s, -op);
+                            else if (hasR)
+                                sth<PF, NT, H16>(B[2 + pol], bo, s, pol ? R1 : R0);
+                            else
+                                sth<PF, NT, H16>(B[0 + pol], bo + s.es, s, op);
+                        }
+                    }
+                    // ---- y faces: ports 4,5 (y-) and 6,7 (y+), sender push ----
+                    {
+                        const bool pecLo = (code & NB_YM) != 0u, pecHi = (code & NB_YP) != 0u;
+        #pragma unroll
+                        for (int pol = 0; pol < 2; ++pol)
+                        {
+                            const float ol = o[4 + pol], oh = o[6 + pol];
+                            if (pec)
+                            {
+                                const float vl = ldh<PF, H16>(A[4 + pol], bo, s), vh = ldh<PF, H16>(A[6 + pol], bo, s);
+                                sth<PF, NT, H16>(B[4 + pol], bo, s, by0 ? rho * vl : vl);
+                                sth<PF, NT, H16>(B[6 + pol], bo, s, by1 ? rho * vh : vh);
+                            }
+                            else
+                            {
+                                if (by0)
+                                    sth<PF, NT, H16>(B[4 + pol], bo, s, rho * ol);
+                                else if (pecLo)
+                                    sth<PF, NT, H16>(B[4 + pol], bo, s, -ol);
+                                else
+                                    sth<PF, NT, H16>(B[6 + pol], bo - by, s, ol);
+                                if (by1)
+                                    sth<PF, NT, H16>(B[6 + pol], bo, s, rho * oh);
+                                else if (pecHi)
+                                    sth<PF, NT, H16>(B[6 + pol], bo, s, -oh);
+                                else
+                                    sth<PF, NT, H16>(B[4 + pol], bo + by, s, oh);
+                            }
+                        }
+                    }
+                    // ---- z faces: ports 8,9 (z-) and 10,11 (z+) ----
+        #pragma unroll
+                    for (int pol = 0; pol < 2; ++pol)
+                    {
+                        const float ol = o[8 + pol];
+                        if (k > k0)
+                        {
+                            // pull: slab k-1 is in registers
+                            const float zp = pol ? zpo1 : zpo0;
+                            if (pec)
+                                sth<PF, NT, H16>(B[8 + pol], bo, s, ldh<PF, H16>(A[8 + pol], bo, s));
+                            else
+                                sth<PF, NT, H16>(B[8 + pol], bo, s, pecPrev ? -ol : zp);
+                            // slab k-1's z+ slot
+                            if (pecPrev)
+                                stp<PF, NT>(B[10 + pol], sp.bo, sp, ldp<PF>(A[10 + pol], sp.bo, sp));
+                            else
+                                stp<PF, NT>(B[10 + pol], sp.bo, sp, pec ? -zp : ol);
+                        }
+                        else if (k == 0)
+                        {
+                            sth<PF, NT, H16>(B[8 + pol], bo, s, rho * (pec ? ldh<PF, H16>(A[8 + pol], bo, s) : ol));
+                        }
                         else
-                            sto<NT>(B[8 + pol], bo + bz, oh);
+                        {
+                            const bool pecLo = (code & NB_ZM) != 0u;
+                            if (pec)
+                                sth<PF, NT, H16>(B[8 + pol], bo, s, ldh<PF, H16>(A[8 + pol], bo, s));
+                            else if (pecLo)
+                                sth<PF, NT, H16>(B[8 + pol], bo, s, -ol);
+                            else
+                            {
+                                const SlabRef sm = slab_ref<PF>(a, k - 1, mo - (unsigned)a.Pp, f);
+                                stp<PF, NT>(B[10 + pol], sm.bo, sm, ol);
+                            }
+                        }
+                        if (k == k1 - 1)
+                        {
+                            const float oh = o[10 + pol];
+                            if (k == a.nz - 1)
+                                sth<PF, NT, H16>(B[10 + pol], bo, s,
+                                            rho * (pec ? ldh<PF, H16>(A[10 + pol], bo, s) : oh));
+                            else
+                            {
+                                const bool pecHi = (code & NB_ZP) != 0u;
+                                if (pec)
+                                    sth<PF, NT, H16>(B[10 + pol], bo, s, ldh<PF, H16>(A[10 + pol], bo, s));
+                                else if (pecHi)
+                                    sth<PF, NT, H16>(B[10 + pol], bo, s, -oh);
+                                else
+                                {
+                                    const SlabRef sq = slab_ref<PF>(a, k + 1, mo + (unsigned)a.Pp, f);
+                                    stp<PF, NT>(B[8 + pol], sq.bo, sq, oh);
+                                }
+                            }
+                        }
                     }
                 }
-            }
-        }
-        zpo0 = o[10];
-        zpo1 = o[11];
-        pecPrev = pec;
+                if (PF != PF_F32)
+                {
+                    float wm = mc;
+        #pragma unroll
+                    for (int off = kWave / 2; off > 0; off >>= 1)
+                        wm = fmaxf(wm, __shfl_xor(wm, off, kWave));
+                    if (lane == 0)
+                        a.mpart[(size_t)k * a.strips + t] = wm;
+                }
+                zpo0 = o[10];
+                zpo1 = o[11];
+                pecPrev = pec;
+                sp = s;
+        };
+        if (PF != PF_F32 && s.h16)
+            slab_body(std::true_type{});
+        else
+            slab_body(std::false_type{});
     }
 }
 
-template <bool STUBS, bool ENERGY, bool NT>
+// ---------------------------------------------------------------------------
+// 16-bit formats. The pulses live in 6 pair planes, plane q holding ports
+// 2q and 2q+1 (the two polarisations of one face) of every cell: a 16-bit
+// slab stores the pair as one dword (two codes), an FP32 slab as a float2.
+// Every store of the step writes both polarisations of one face of one cell
+// with the same value rule, so a lane moves one dword (or 8 bytes) per
+// face: 6 loads and 6 stores per cell, as many bytes per instruction as the
+// FP32 layout.
+// ---------------------------------------------------------------------------
+struct Slab16
+{
+    unsigned bo, es; // byte offset of the lane's cell pair inside a plane, bytes per cell
+    bool h16;
+    float ds, isc;
+};
+__device__ __forceinline__ Slab16 slab16(const StepArgs &a, int k, int f)
+{
+    const uint2 s = a.slab[k];
+    const bool h = __builtin_amdgcn_readfirstlane(s.y) != 0u; // one format per wave
+    const unsigned es = h ? 4u : 8u;
+    return {s.x + (unsigned)f * es, es, h, a.dsc[k], a.esc[k]};
+}
+template <int PF, bool H16>
+__device__ __forceinline__ void ldpair(const char *plane, unsigned off, const Slab16 &s, float &x,
+                                       float &y)
+{
+    if (H16)
+    {
+        const unsigned u = *reinterpret_cast<const unsigned *>(plane + off);
+        x = dec16<PF>(u & 0xffffu) * s.ds;
+        y = dec16<PF>(u >> 16) * s.ds;
+    }
+    else
+    {
+        const float2 w = *reinterpret_cast<const float2 *>(plane + off);
+        x = w.x;
+        y = w.y;
+    }
+}
+template <int PF, bool H16>
+__device__ __forceinline__ void stpair(char *plane, unsigned off, const Slab16 &s, float x, float y)
+{
+    if (H16)
+        *reinterpret_cast<unsigned *>(plane + off) =
+            (unsigned)enc16<PF>(x * s.isc) | ((unsigned)enc16<PF>(y * s.isc) << 16);
+    else
+        *reinterpret_cast<float2 *>(plane + off) = make_float2(x, y);
+}
+// format known only at run time (another slab; the branch is wave-uniform)
+template <int PF>
+__device__ __forceinline__ void ldpair_d(const char *plane, unsigned off, const Slab16 &s, float &x,
+                                         float &y)
+{
+    if (s.h16)
+        ldpair<PF, true>(plane, off, s, x, y);
+    else
+        ldpair<PF, false>(plane, off, s, x, y);
+}
+template <int PF>
+__device__ __forceinline__ void stpair_d(char *plane, unsigned off, const Slab16 &s, float x, float y)
+{
+    if (s.h16)
+        stpair<PF, true>(plane, off, s, x, y);
+    else
+        stpair<PF, false>(plane, off, s, x, y);
+}
+
+// The fused step of strip_task with pair planes and per-slab formats (same
+// delivery rules, same FP32 arithmetic, values rounded where stored).
+template <bool STUBS, bool ENERGY, int PF>
+__device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int lane, double &esum)
+{
+    const int tz = task / a.strips, t = task - tz * a.strips;
+    const int f = t * kWave + lane;
+    const bool act = f < a.nxy;
+    const int j = f / a.nx, i = f - j * a.nx;
+    const int k0 = tz * a.zc, k1 = min(a.nz, k0 + a.zc);
+    const float rho = a.rho;
+    const char *A[6];
+    char *B[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+    {
+        A[q] = a.A + q * a.PB;
+        B[q] = a.B + q * a.PB;
+    }
+    const bool hasL = act && lane > 0 && i > 0;
+    const bool hasR = act && lane < kWave - 1 && i < a.nx - 1;
+    const bool bx0 = i == 0, bx1 = i == a.nx - 1, by0 = j == 0, by1 = j == a.ny - 1;
+    float zpo0 = 0.f, zpo1 = 0.f;
+    bool pecPrev = false;
+    unsigned mo = (unsigned)(k0 * a.Pp + f);
+    Slab16 sp = {0u, 8u, false, 1.0f, 1.0f};
+
+    for (int k = k0; k < k1; ++k, mo += (unsigned)a.Pp)
+    {
+        const Slab16 s = slab16(a, k, f);
+        auto body = [&](auto h16_tag) {
+            constexpr bool H = decltype(h16_tag)::value;
+            const unsigned bo = s.bo;
+            const unsigned by = (unsigned)a.nx * s.es;
+            float o[12];
+            float mc = 0.f;
+            bool pec = true;
+            unsigned code = 0;
+            if (act)
+            {
+                code = ldc(a.code, mo);
+                const unsigned m = code & 0xFFu;
+                pec = (m == 1u);
+                float v[12];
+#pragma unroll
+                for (int q = 0; q < 6; ++q)
+                    ldpair<PF, H>(A[q], bo, s, v[2 * q], v[2 * q + 1]);
+                if (ENERGY)
+                {
+#pragma unroll
+                    for (int p = 0; p < 12; ++p)
+                        esum += (double)v[p] * (double)v[p];
+                }
+                if (STUBS && m >= 2u)
+                {
+                    float *S0 = a.Vs, *S1 = a.Vs + a.P, *S2 = a.Vs + 2 * a.P;
+                    const unsigned bs = mo * 4u;
+                    float st[3] = {ldo((const char *)S0, bs), ldo((const char *)S1, bs),
+                                   ldo((const char *)S2, bs)};
+                    const float4 pr = a.props[m - 2u];
+                    scn::scatter_diel(v, st, pr.x, pr.z, o);
+                    sto<false>((char *)S0, bs, st[0]);
+                    sto<false>((char *)S1, bs, st[1]);
+                    sto<false>((char *)S2, bs, st[2]);
+                    mc = fmaxf(fmaxf(fabsf(st[0]), fabsf(st[1])), fabsf(st[2]));
+                }
+                else
+                    scn::scatter_air(v, o);
+#pragma unroll
+                for (int p = 0; p < 12; ++p)
+                    mc = fmaxf(mc, fabsf(pec ? v[p] : o[p]));
+                if (pec)
+                {
+                    // a PEC cell keeps its incident pulses: o carries them
+#pragma unroll
+                    for (int p = 0; p < 12; ++p)
+                        o[p] = v[p];
+                }
+            }
+            else
+            {
+#pragma unroll
+                for (int p = 0; p < 12; ++p)
+                    o[p] = 0.f;
+            }
+            const float L0 = __shfl_up(o[2], 1, kWave), L1 = __shfl_up(o[3], 1, kWave);
+            const float R0 = __shfl_down(o[0], 1, kWave), R1 = __shfl_down(o[1], 1, kWave);
+            if (act)
+            {
+                // ---- x faces: pair 0 (x-: ports 0,1), pair 1 (x+: ports 2,3) ----
+                const bool pecL = (code & NB_XM) != 0u, pecR = (code & NB_XP) != 0u;
+                if (pec)
+                    stpair<PF, H>(B[0], bo, s, bx0 ? rho * o[0] : o[0], bx0 ? rho * o[1] : o[1]);
+                else if (bx0)
+                    stpair<PF, H>(B[0], bo, s, rho * o[0], rho * o[1]);
+                else if (pecL)
+                    stpair<PF, H>(B[0], bo, s, -o[0], -o[1]);
+                else if (hasL)
+                    stpair<PF, H>(B[0], bo, s, L0, L1);
+                else
+                    stpair<PF, H>(B[1], bo - s.es, s, o[0], o[1]);
+                if (pec)
+                    stpair<PF, H>(B[1], bo, s, bx1 ? rho * o[2] : o[2], bx1 ? rho * o[3] : o[3]);
+                else if (bx1)
+                    stpair<PF, H>(B[1], bo, s, rho * o[2], rho * o[3]);
+                else if (pecR)
+                    stpair<PF, H>(B[1], bo, s, -o[2], -o[3]);
+                else if (hasR)
+                    stpair<PF, H>(B[1], bo, s, R0, R1);
+                else
+                    stpair<PF, H>(B[0], bo + s.es, s, o[2], o[3]);
+                // ---- y faces: pair 2 (y-: ports 4,5), pair 3 (y+: ports 6,7) ----
+                const bool pecLo = (code & NB_YM) != 0u, pecHi = (code & NB_YP) != 0u;
+                if (pec)
+                {
+                    stpair<PF, H>(B[2], bo, s, by0 ? rho * o[4] : o[4], by0 ? rho * o[5] : o[5]);
+                    stpair<PF, H>(B[3], bo, s, by1 ? rho * o[6] : o[6], by1 ? rho * o[7] : o[7]);
+                }
+                else
+                {
+                    if (by0)
+                        stpair<PF, H>(B[2], bo, s, rho * o[4], rho * o[5]);
+                    else if (pecLo)
+                        stpair<PF, H>(B[2], bo, s, -o[4], -o[5]);
+                    else
+                        stpair<PF, H>(B[3], bo - by, s, o[4], o[5]);
+                    if (by1)
+                        stpair<PF, H>(B[3], bo, s, rho * o[6], rho * o[7]);
+                    else if (pecHi)
+                        stpair<PF, H>(B[3], bo, s, -o[6], -o[7]);
+                    else
+                        stpair<PF, H>(B[2], bo + by, s, o[6], o[7]);
+                }
+                // ---- z faces: pair 4 (z-: ports 8,9), pair 5 (z+: ports 10,11) ----
+                if (k > k0)
+                {
+                    // pull: slab k-1's reflected z+ pulses are in registers
+                    if (pec)
+                        stpair<PF, H>(B[4], bo, s, o[8], o[9]);
+                    else
+                        stpair<PF, H>(B[4], bo, s, pecPrev ? -o[8] : zpo0, pecPrev ? -o[9] : zpo1);
+                    // slab k-1's z+ slot: its kept pulses if it is PEC (then
+                    // zpo holds them), else what this cell sends down
+                    if (pecPrev)
+                        stpair_d<PF>(B[5], sp.bo, sp, zpo0, zpo1);
+                    else
+                        stpair_d<PF>(B[5], sp.bo, sp, pec ? -zpo0 : o[8], pec ? -zpo1 : o[9]);
+                }
+                else if (k == 0)
+                    stpair<PF, H>(B[4], bo, s, rho * o[8], rho * o[9]);
+                else
+                {
+                    const bool pecZ = (code & NB_ZM) != 0u;
+                    if (pec)
+                        stpair<PF, H>(B[4], bo, s, o[8], o[9]);
+                    else if (pecZ)
+                        stpair<PF, H>(B[4], bo, s, -o[8], -o[9]);
+                    else
+                    {
+                        const Slab16 sm = slab16(a, k - 1, f);
+                        stpair_d<PF>(B[5], sm.bo, sm, o[8], o[9]);
+                    }
+                }
+                if (k == k1 - 1)
+                {
+                    if (k == a.nz - 1)
+                        stpair<PF, H>(B[5], bo, s, rho * o[10], rho * o[11]);
+                    else
+                    {
+                        const bool pecZ = (code & NB_ZP) != 0u;
+                        if (pec)
+                            stpair<PF, H>(B[5], bo, s, o[10], o[11]);
+                        else if (pecZ)
+                            stpair<PF, H>(B[5], bo, s, -o[10], -o[11]);
+                        else
+                        {
+                            const Slab16 sq = slab16(a, k + 1, f);
+                            stpair_d<PF>(B[4], sq.bo, sq, o[10], o[11]);
+                        }
+                    }
+                }
+            }
+            float wm = mc;
+#pragma unroll
+            for (int off = kWave / 2; off > 0; off >>= 1)
+                wm = fmaxf(wm, __shfl_xor(wm, off, kWave));
+            if (lane == 0)
+                a.mpart[(size_t)k * a.strips + t] = wm;
+            zpo0 = o[10];
+            zpo1 = o[11];
+            pecPrev = pec;
+        };
+        if (s.h16)
+            body(std::true_type{});
+        else
+            body(std::false_type{});
+        sp = s;
+    }
+}
+
+template <bool STUBS, bool ENERGY, bool NT, int PF>
 __global__ void __launch_bounds__(256) k_step(StepArgs a)
 {
     const int lane = threadIdx.x & (kWave - 1);
@@ -327,7 +718,12 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a)
     {
         const int task = g * wpb + wave;
         if (task < a.tasks)
-            strip_task<STUBS, ENERGY, NT>(a, task, lane, esum);
+        {
+            if constexpr (PF == PF_F32)
+                strip_task<STUBS, ENERGY, NT, PF>(a, task, lane, esum);
+            else
+                strip_task16<STUBS, ENERGY, PF>(a, task, lane, esum);
+        }
     }
     if (ENERGY)
     {
@@ -338,19 +734,147 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a)
     }
 }
 
-
-// Soft-source injection (TlmSolver.cpp:487-505). One thread per distinct
-// cell adds its sources in list order, so every slot sees the same
-// sequence of additions as the reference.
-__global__ void k_inject(float *A, long long P, const long long *ucell, const int *ubeg,
-                         const int *spol, const float *samp, int nuniq, const float *sv_tab, int n)
+// Per-slab maximum of the wave records of one step (one workgroup per slab).
+static __global__ void k_slab_max(const float *mpart, int strips, float *mreg)
 {
-    const int u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= nuniq)
-        return;
-    const float sv = sv_tab[n];
-    const long long c = ucell[u];
-    for (int s = ubeg[u]; s < ubeg[u + 1]; ++s)
+    __shared__ float s[256];
+    const float *row = mpart + (size_t)blockIdx.x * strips;
+    float m = 0.f;
+    for (int t = threadIdx.x; t < strips; t += blockDim.x)
+        m = fmaxf(m, row[t]);
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1)
+    {
+        if ((int)threadIdx.x < w)
+            s[threadIdx.x] = fmaxf(s[threadIdx.x], s[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        mreg[blockIdx.x] = s[0];
+}
+
+// Pulse access by padded cell index for the small kernels (injection,
+// monitors, conversions), any format.
+struct PulseView
+{
+    char *base;
+    long long PB;     // bytes per plane
+    long long Pp;     // slab pitch (cells)
+    int pf;           // PF_*
+    const uint2 *slab; // 16-bit formats: per slab (byte offset, 16-bit flag)
+    const float *sc;  // per slab 2^E (current step)
+    const float *isc; // per slab 2^-E
+};
+// FP32 contexts: 12 planes of floats. 16-bit contexts: 6 pair planes
+// (plane p/2, half p&1), a 16-bit slab holding two codes per cell, an FP32
+// slab a float2.
+__device__ __forceinline__ float pv_get(const PulseView &v, int p, long long c)
+{
+    if (v.pf == PF_F32)
+        return reinterpret_cast<const float *>(v.base + p * v.PB)[c];
+    const char *plane = v.base + (p >> 1) * v.PB;
+    const long long k = c / v.Pp, f = c - k * v.Pp;
+    const uint2 s = v.slab[k];
+    if (!s.y)
+        return reinterpret_cast<const float *>(plane + s.x)[2 * f + (p & 1)];
+    const unsigned h = reinterpret_cast<const unsigned short *>(plane + s.x)[2 * f + (p & 1)];
+    return (v.pf == PF_F16 ? dec16<PF_F16>(h) : dec16<PF_BF16>(h)) * v.sc[k];
+}
+// Stores x; returns |x * 2^-E| for a 16-bit slot (0 for FP32) so callers can
+// check the range.
+__device__ __forceinline__ float pv_set(const PulseView &v, int p, long long c, float x)
+{
+    if (v.pf == PF_F32)
+    {
+        reinterpret_cast<float *>(v.base + p * v.PB)[c] = x;
+        return 0.f;
+    }
+    char *plane = v.base + (p >> 1) * v.PB;
+    const long long k = c / v.Pp, f = c - k * v.Pp;
+    const uint2 s = v.slab[k];
+    if (!s.y)
+    {
+        reinterpret_cast<float *>(plane + s.x)[2 * f + (p & 1)] = x;
+        return 0.f;
+    }
+    const float y = x * v.isc[k];
+    reinterpret_cast<unsigned short *>(plane + s.x)[2 * f + (p & 1)] =
+        v.pf == PF_F16 ? enc16<PF_F16>(y) : enc16<PF_BF16>(y);
+    return fabsf(y);
+}
+
+// Slab exponent for a bound: bound * 2^-E < 2^15, E in [-100, 100]
+// (identical integer steps on the host and in the oracle).
+__host__ __device__ inline int plan_exponent(float bound)
+{
+    if (!(bound > 0.0f))
+        return 0;
+    unsigned u;
+    __builtin_memcpy(&u, &bound, 4);
+    const int be = (int)((u >> 23) & 0xffu);
+    if (be == 0)
+        return -100;
+    int e = be - 126 - 15;
+    return e < -100 ? -100 : (e > 100 ? 100 : e);
+}
+__host__ __device__ inline float pow2_float(int e)
+{
+    unsigned u = (unsigned)(e + 127) << 23;
+    float f;
+    __builtin_memcpy(&f, &u, 4);
+    return f;
+}
+
+// Scale planning of the 16-bit formats, run before step n:
+//   bound_k = (rp*6) * (rp*max_{|d|<=2} M[k+d] + max_{|d|<=1} Q[k+d]*|sv_n|)
+//             + Q_k*|sv_{n+1}|
+// bounds every value stored into slab k by step n (one step multiplies the
+// max norm by at most 5; a pulse moves one slab per step), M = the per-slab
+// maximum the cells sent at step n-1, Q = 0.5 * sum |amp| of the slab's soft
+// sources. Also checks that the values step n-1 stored fit their codes.
+struct PlanArgs
+{
+    int nz;
+    const float *mreg, *Q;
+    const uint2 *slab;
+    const float *sv;
+    int n, nsteps;
+    float rp, fmax;
+    const float *isc_cur;
+    float *sc_next, *isc_next;
+    int *flag;
+};
+__device__ __forceinline__ void plan_slab(const PlanArgs &p, int k)
+{
+    float Mn = 0.f, Qn = 0.f, M1 = 0.f;
+    for (int d = -2; d <= 2; ++d)
+        if (k + d >= 0 && k + d < p.nz)
+        {
+            Mn = fmaxf(Mn, p.mreg[k + d]);
+            if (d >= -1 && d <= 1)
+            {
+                Qn = fmaxf(Qn, p.Q[k + d]);
+                M1 = fmaxf(M1, p.mreg[k + d]);
+            }
+        }
+    const float svn = fabsf(p.sv[p.n]);
+    const float svn1 = (p.n + 1 < p.nsteps) ? fabsf(p.sv[p.n + 1]) : 0.0f;
+    const float b = (p.rp * 6.0f) * ((p.rp * Mn) + (Qn * svn)) + (p.Q[k] * svn1);
+    const int E = plan_exponent(b);
+    p.sc_next[k] = pow2_float(E);
+    p.isc_next[k] = pow2_float(-E);
+    if (p.slab[k].y && (p.rp * M1) * p.isc_cur[k] > p.fmax)
+        p.flag[0] = 1;
+}
+
+// Adds the sources [s0, s1) of one cell in list order (0.5*amp*sv on the four
+// ports polarised along pol); returns the largest |code| a 16-bit slot took.
+__device__ __forceinline__ float inject_cell(const PulseView &pv, long long c, int s0, int s1,
+                                             const int *spol, const float *samp, float sv)
+{
+    float over = 0.f;
+    for (int s = s0; s < s1; ++s)
     {
         const int pol = spol[s];
         const float qv = 0.5f * samp[s] * sv;
@@ -359,23 +883,92 @@ __global__ void k_inject(float *A, long long P, const long long *ucell, const in
             if (axis == pol)
                 continue;
             const int p0 = scn::port_of(axis, 0, pol), p1 = scn::port_of(axis, 1, pol);
-            A[p0 * P + c] += qv;
-            A[p1 * P + c] += qv;
+            over = fmaxf(over, pv_set(pv, p0, c, pv_get(pv, p0, c) + qv));
+            over = fmaxf(over, pv_set(pv, p1, c, pv_get(pv, p1, c) + qv));
         }
     }
+    return over;
 }
 
-__device__ __forceinline__ void gather12(const float *A, long long P, long long c, float v[12])
+// Soft-source injection (TlmSolver.cpp:487-505). One thread per distinct
+// cell adds its sources in list order, so every slot sees the same
+// sequence of additions as the reference (a 16-bit slot is rounded after
+// each addition). Threads past the sources plan the slab scales of the
+// next step (16-bit formats, pl.nz > 0).
+static __global__ void k_inject(PulseView pv, const long long *ucell, const int *ubeg, const int *spol,
+                         const float *samp, int nuniq, const float *sv_tab, int n, PlanArgs pl)
+{
+    int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nuniq)
+    {
+        u -= nuniq;
+        if (u < pl.nz)
+            plan_slab(pl, u);
+        return;
+    }
+    const float over = inject_cell(pv, ucell[u], ubeg[u], ubeg[u + 1], spol, samp, sv_tab[n]);
+    if (pl.flag && over > pl.fmax)
+        pl.flag[0] = 1;
+}
+
+__device__ __forceinline__ void gather12(const PulseView &pv, long long c, float v[12])
 {
 #pragma unroll
     for (int p = 0; p < 12; ++p)
-        v[p] = A[p * P + c];
+        v[p] = pv_get(pv, p, c);
+}
+
+// Port record of step n (TlmSolver.cpp:677-703): sequential float sum of
+// V_j over the port cells and I from Ampere's loop round the middle cell.
+__device__ __forceinline__ void monitor_port(const PulseView &pv, int id, int n, int np,
+                                             const long long *pcells, const int *poff,
+                                             const int *ppol, const long long *ploop, float *prec)
+{
+    const int pol = ppol[id];
+    float vsum = 0.0f, ival = 0.0f;
+    const int b = poff[id], e = poff[id + 1];
+    for (int s = b; s < e; ++s)
+    {
+        float v[12];
+        gather12(pv, pcells[s], v);
+        vsum += scn::vj(v, pol);
+    }
+    if (e > b)
+    {
+        // Ampere loop round the middle cell (TlmSolver.cpp:687-699)
+        const int t1 = (pol + 1) % 3, t2 = (pol + 2) % 3; // CYC_I, CYC_J
+        const long long *L = ploop + 4 * (size_t)id;
+        float v[12], zsum = 0.0f;
+        gather12(pv, L[0], v);
+        zsum += scn::zi(v, t2);
+        gather12(pv, L[1], v);
+        zsum -= scn::zi(v, t2);
+        gather12(pv, L[2], v);
+        zsum -= scn::zi(v, t1);
+        gather12(pv, L[3], v);
+        zsum += scn::zi(v, t1);
+        ival = -2.0f * zsum / ETA0;
+    }
+    float *rec = prec + ((size_t)n * np + id) * 2;
+    rec[0] = vsum;
+    rec[1] = ival;
+}
+
+// E probe record of step n: (-1/dl) * V_j (TlmSolver.cpp:705-713).
+__device__ __forceinline__ void monitor_probe(const PulseView &pv, int id, int n, int npr,
+                                              const long long *prcell, float inv_dl, float *prrec)
+{
+    float v[12];
+    gather12(pv, prcell[id], v);
+    float *rec = prrec + ((size_t)n * npr + id) * 3;
+    rec[0] = inv_dl * scn::vj(v, 0);
+    rec[1] = inv_dl * scn::vj(v, 1);
+    rec[2] = inv_dl * scn::vj(v, 2);
 }
 
 struct MonitorArgs
 {
-    const float *A;
-    long long P;
+    PulseView pv;
     int n, settle;
     const float *cph, *sph;
     // Js faces
@@ -409,14 +1002,14 @@ struct MonitorArgs
 
 // Monitors on the post-injection incident pulses (TlmSolver::monitors,
 // TlmSolver.cpp:616-713), one thread per face / port / probe.
-__global__ void k_monitor(MonitorArgs a)
+static __global__ void k_monitor(MonitorArgs a)
 {
     int id = blockIdx.x * blockDim.x + threadIdx.x;
     const bool doDft = a.n >= a.settle;
     if (id < a.nf)
     {
         float v[12];
-        gather12(a.A, a.P, a.fcell[id], v);
+        gather12(a.pv, a.fcell[id], v);
         const int ax = a.faxis[id];
         const float h1 = scn::zi(v, scn::pol_axis(ax, 0)), h2 = scn::zi(v, scn::pol_axis(ax, 1));
         a.js_inst[id] = sqrtf(h1 * h1 + h2 * h2);
@@ -437,7 +1030,7 @@ __global__ void k_monitor(MonitorArgs a)
         if (!doDft)
             return;
         float v[12];
-        gather12(a.A, a.P, a.hcell[id], v);
+        gather12(a.pv, a.hcell[id], v);
         const int ax = a.haxis[id];
         const int t1 = scn::pol_axis(ax, 0), t2 = scn::pol_axis(ax, 1);
         const float e1 = -scn::vj(v, t1), e2 = -scn::vj(v, t2);
@@ -457,45 +1050,13 @@ __global__ void k_monitor(MonitorArgs a)
     id -= a.nh;
     if (id < a.np)
     {
-        const int pol = a.ppol[id];
-        float vsum = 0.0f, ival = 0.0f;
-        const int b = a.poff[id], e = a.poff[id + 1];
-        for (int s = b; s < e; ++s)
-        {
-            float v[12];
-            gather12(a.A, a.P, a.pcells[s], v);
-            vsum += scn::vj(v, pol);
-        }
-        if (e > b)
-        {
-            // Ampere loop round the middle cell (TlmSolver.cpp:687-699)
-            const int t1 = (pol + 1) % 3, t2 = (pol + 2) % 3; // CYC_I, CYC_J
-            const long long *L = a.ploop + 4 * (size_t)id;
-            float v[12], zsum = 0.0f;
-            gather12(a.A, a.P, L[0], v);
-            zsum += scn::zi(v, t2);
-            gather12(a.A, a.P, L[1], v);
-            zsum -= scn::zi(v, t2);
-            gather12(a.A, a.P, L[2], v);
-            zsum -= scn::zi(v, t1);
-            gather12(a.A, a.P, L[3], v);
-            zsum += scn::zi(v, t1);
-            ival = -2.0f * zsum / ETA0;
-        }
-        float *rec = a.prec + ((size_t)a.n * a.np + id) * 2;
-        rec[0] = vsum;
-        rec[1] = ival;
+        monitor_port(a.pv, id, a.n, a.np, a.pcells, a.poff, a.ppol, a.ploop, a.prec);
         return;
     }
     id -= a.np;
     if (id < a.npr)
     {
-        float v[12];
-        gather12(a.A, a.P, a.prcell[id], v);
-        float *rec = a.prrec + ((size_t)a.n * a.npr + id) * 3;
-        rec[0] = a.inv_dl * scn::vj(v, 0);
-        rec[1] = a.inv_dl * scn::vj(v, 1);
-        rec[2] = a.inv_dl * scn::vj(v, 2);
+        monitor_probe(a.pv, id, a.n, a.npr, a.prcell, a.inv_dl, a.prrec);
         return;
     }
     id -= a.npr;
@@ -513,7 +1074,7 @@ __global__ void k_monitor(MonitorArgs a)
         if ((a.code[c] & 0xFFu) != 1u)
         {
             float v[12];
-            gather12(a.A, a.P, c, v);
+            gather12(a.pv, c, v);
             float e2 = 0.0f;
             for (int j = 0; j < 3; ++j)
             {
@@ -528,7 +1089,7 @@ __global__ void k_monitor(MonitorArgs a)
 
 // First stage of the energy sum: block b adds partials [b*chunk, (b+1)*chunk)
 // in a fixed order (so the result does not depend on scheduling).
-__global__ void k_energy_chunks(const double *part, long long nparts, long long chunk, double *out)
+static __global__ void k_energy_chunks(const double *part, long long nparts, long long chunk, double *out)
 {
     __shared__ double s[256];
     const long long lo = blockIdx.x * chunk, hi = min(nparts, lo + chunk);
@@ -548,7 +1109,7 @@ __global__ void k_energy_chunks(const double *part, long long nparts, long long 
 }
 
 // Sum of the energy partials of one step in a fixed order.
-__global__ void k_energy_reduce(const double *part, int nparts, double *out)
+static __global__ void k_energy_reduce(const double *part, int nparts, double *out)
 {
     __shared__ double s[256];
     double acc = 0.0;
@@ -567,7 +1128,7 @@ __global__ void k_energy_reduce(const double *part, int nparts, double *out)
 }
 
 // Running FP64 DFT of the probe records over steps [n0, n1), in step order.
-__global__ void k_probe_dft(const float *rec, int npr, const double *ctab, const double *stab,
+static __global__ void k_probe_dft(const float *rec, int npr, const double *ctab, const double *stab,
                             int nfreq, int n0, int n1, double *re, double *im)
 {
     const int id = blockIdx.x * blockDim.x + threadIdx.x;
@@ -586,8 +1147,8 @@ __global__ void k_probe_dft(const float *rec, int npr, const double *ctab, const
 }
 
 // AoS (12 or 3 floats per cell, unpadded) <-> SoA padded planes, for the
-// cells [c0, c0 + count) of the unpadded index space.
-__global__ void k_aos_to_soa(const float *aos, float *soa, long long P, int ncomp, long long nxy,
+// cells [c0, c0 + count) of the unpadded index space. Stubs are always FP32.
+static __global__ void k_aos_to_soa(const float *aos, float *soa, long long P, int ncomp, long long nxy,
                              long long Pp, long long c0, long long count)
 {
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -600,7 +1161,7 @@ __global__ void k_aos_to_soa(const float *aos, float *soa, long long P, int ncom
         soa[p * P + pc] = aos[id * ncomp + p];
 }
 
-__global__ void k_soa_to_aos(const float *soa, float *aos, long long P, int ncomp, long long nxy,
+static __global__ void k_soa_to_aos(const float *soa, float *aos, long long P, int ncomp, long long nxy,
                              long long Pp, long long c0, long long count)
 {
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -613,10 +1174,47 @@ __global__ void k_soa_to_aos(const float *soa, float *aos, long long P, int ncom
         aos[id * ncomp + p] = soa[p * P + pc];
 }
 
+// Pulses (12 per cell) in any storage format.
+static __global__ void k_pulses_in(const float *aos, PulseView pv, long long nxy, long long c0,
+                            long long count)
+{
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= count)
+        return;
+    const long long cell = c0 + id;
+    const long long k = cell / nxy;
+    const long long pc = k * pv.Pp + (cell - k * nxy);
+    for (int p = 0; p < 12; ++p)
+        pv_set(pv, p, pc, aos[id * 12 + p]);
+}
+
+static __global__ void k_pulses_out(PulseView pv, float *aos, long long nxy, long long c0, long long count)
+{
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= count)
+        return;
+    const long long cell = c0 + id;
+    const long long k = cell / nxy;
+    const long long pc = k * pv.Pp + (cell - k * nxy);
+    for (int p = 0; p < 12; ++p)
+        aos[id * 12 + p] = pv_get(pv, p, pc);
+}
+
+// Device rounding of the 16-bit formats (v * inv_scale), for checking the
+// conversion instructions against the IEEE definition.
+static __global__ void k_encode(int pf, const float *in, float inv_scale, unsigned short *out, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const float y = in[i] * inv_scale;
+    out[i] = pf == PF_F16 ? enc16<PF_F16>(y) : enc16<PF_BF16>(y);
+}
+
 // Scattering matrix columns of one node (air or stub-loaded) through the
 // device scatter: column c is the response to a unit pulse on port c
 // (c < 12) or stub c-12.
-__global__ void k_scatter_columns(float yh, float den, int diel, float *S /*[15][15] row-major*/)
+static __global__ void k_scatter_columns(float yh, float den, int diel, float *S /*[15][15] row-major*/)
 {
     const int c = threadIdx.x;
     if (c >= 15)
@@ -638,7 +1236,7 @@ __global__ void k_scatter_columns(float yh, float den, int diel, float *S /*[15]
 
 // Plain grid-stride copy: the reference kernel for the measured device-copy
 // bandwidth and for scaling the FETCH_SIZE / WRITE_SIZE counters.
-__global__ void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, long long n4)
+static __global__ void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, long long n4)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
          i += (long long)gridDim.x * blockDim.x)

@@ -61,6 +61,8 @@ class Scene:
     plane_index: int = 0
     record_every: int = 0         # playback frames (TlmConfig::recordEvery), 0 off
     faces: bool = True            # PEC surface-current monitor (always on in the reference)
+    pulse: str = "f32"            # pulse storage: "f32", "f16" or "bf16"
+    fp32_halo: int = 8            # 16-bit storage: FP32 slabs each side of a source slab (< 0 none)
     V0: np.ndarray | None = None  # optional initial pulses (12 per cell, AoS)
     Vs0: np.ndarray | None = None
     name: str = "scene"
@@ -239,3 +241,35 @@ def selftest_dipole(waveform=WF_GAUSS_SINE, extra_steps=0, diel_sphere=False) ->
               huy_offset=pad // 2, name="selftest_dipole")
     s.ports = [(sorted(cells), 2, 1.0)]
     return s
+
+
+def batch_dipoles(count=256, lo=32, hi=128, steps=(500, 2000), seed=3, nprobes=2,
+                  length=(0.3, 0.7), background_boxes=8, pulse="f32") -> list:
+    """Independent grids for the batched run: each axis log-uniform in
+    [lo, hi] cells, a centre-fed z wire dipole of length U[length]*nz over a
+    seeded random-dielectric background (the dielectric-box generator scaled
+    to the grid), steps uniform in [steps], `nprobes` seeded E probes, port
+    V/I recorded. No surface-current monitor (batched members record probes
+    and ports)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for g in range(count):
+        nx, ny, nz = (int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+                      for _ in range(3))
+        frac = float(rng.uniform(*length))
+        st = int(rng.integers(steps[0], steps[1] + 1))
+        sc = dipole_scene(nx, ny, nz, length_frac=frac, steps=st,
+                          background_boxes=background_boxes, seed=int(rng.integers(1 << 31)))
+        sc.faces = False
+        sc.pulse = pulse
+        probes = []
+        while len(probes) < nprobes:
+            i, j, k = int(rng.integers(1, nx - 1)), int(rng.integers(1, ny - 1)), int(
+                rng.integers(1, nz - 1))
+            c = sc.cell(i, j, k)
+            if sc.mat[c] != MAT_PEC and c not in probes:
+                probes.append(c)
+        sc.probes = np.array(probes, np.int64)
+        sc.name = f"batch{g}_{nx}x{ny}x{nz}"
+        out.append(sc)
+    return out

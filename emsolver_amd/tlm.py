@@ -105,12 +105,18 @@ def huygens_faces(nx, ny, nz, mat, offset):
 class TlmDevice:
     """One device context (``tlm_ctx``)."""
 
-    def __init__(self, nx, ny, nz, dl, rho=0.0, pulse="f32", device=0):
+    def __init__(self, nx, ny, nz, dl, rho=0.0, pulse="f32", device=0, fp32_halo=8):
+        """pulse: "f32", "f16" or "bf16" storage; fp32_halo: for the 16-bit
+        formats, z-slabs on each side of a source slab kept in FP32 (< 0: none)."""
         self.lib = _native.load()
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.n = self.nx * self.ny * self.nz
+        self.pulse = pulse
+        halo = -1 if fp32_halo < 0 else int(fp32_halo)
+        if halo == 0:
+            raise ValueError("fp32_halo must be >= 1, or negative for none")
         d = _native.GridDesc(self.nx, self.ny, self.nz, float(dl), float(rho),
-                             PULSE_TYPES[pulse], int(device), 0)
+                             PULSE_TYPES[pulse], int(device), halo)
         h = C.c_void_p()
         rc = self.lib.tlm_create(C.byref(d), C.byref(h))
         if rc != 0:
@@ -291,6 +297,18 @@ class TlmDevice:
                   "tlm_read_state")
         return V, Vs
 
+    def read_scales(self):
+        """16-bit formats: per z-slab (scale 2^E of the current step, largest
+        |value| sent in the last step, storage format); None otherwise."""
+        sc = np.zeros(self.nz, np.float32)
+        mx = np.zeros(self.nz, np.float32)
+        fm = np.zeros(self.nz, np.int32)
+        n = self.lib.tlm_read_scales(self.h, ptr(sc, C.c_float), ptr(mx, C.c_float),
+                                     ptr(fm, C.c_int32))
+        if n < 0:
+            self._chk(n, "tlm_read_scales")
+        return (sc, mx, fm) if n > 0 else None
+
     def read_probes(self, step0, nsteps):
         e = np.empty((nsteps, self.nprobe, 3), np.float32)
         self._chk(self.lib.tlm_read_probes(self.h, step0, nsteps, ptr(e, C.c_float)),
@@ -356,7 +374,8 @@ class SceneResult:
 
 def open_scene(scene, device=0) -> TlmDevice:
     """Create a context and load a Scene into it (no steps run)."""
-    dev = TlmDevice(scene.nx, scene.ny, scene.nz, scene.dl, scene.rho, device=device)
+    dev = TlmDevice(scene.nx, scene.ny, scene.nz, scene.dl, scene.rho, pulse=scene.pulse,
+                    device=device, fp32_halo=scene.fp32_halo)
     yh, gh = stub_admittances(scene.epsr, scene.sigma, scene.dl)
     dev.upload(scene.mat, yh, gh, scene.V0, scene.Vs0)
     dt = timestep(scene.dl)
@@ -439,3 +458,141 @@ def run_scene(scene, steps=None, device=0, state=True, block=256) -> SceneResult
             n += m
         dev.sync()
         return collect(dev, scene, steps, state)
+
+
+def device_encode(fmt, values, inv_scale=1.0, device=0):
+    """16-bit codes of values * inv_scale rounded by the device conversion
+    (fmt "f16" or "bf16")."""
+    lib = _native.load()
+    v = np.ascontiguousarray(values, np.float32).reshape(-1)
+    out = np.zeros(v.size, np.uint16)
+    rc = lib.tlm_device_encode(int(device), PULSE_TYPES[fmt], ptr(v, C.c_float),
+                               C.c_float(inv_scale), ptr(out, C.c_uint16), v.size)
+    if rc != 0:
+        raise TlmError(f"tlm_device_encode failed ({rc})")
+    return out
+
+
+def device_memory(device=0):
+    """(free, total) bytes of device memory."""
+    lib = _native.load()
+    f, t = C.c_size_t(0), C.c_size_t(0)
+    rc = lib.tlm_device_memory(int(device), C.byref(f), C.byref(t))
+    if rc != 0:
+        raise TlmError(f"tlm_device_memory failed ({rc})")
+    return f.value, t.value
+
+
+def estimate_bytes(nx, ny, nz, pulse="f32", stubs=True) -> int:
+    """Device bytes a context of this shape allocates (monitors excluded)."""
+    lib = _native.load()
+    d = _native.GridDesc(int(nx), int(ny), int(nz), 0.01, 0.0, PULSE_TYPES[pulse], 0, 0)
+    return int(lib.tlm_estimate_bytes(C.byref(d), int(bool(stubs))))
+
+
+class TlmBatch:
+    """Many contexts advanced by the same launches (``tlm_batch``)."""
+
+    def __init__(self, device=0):
+        self.lib = _native.load()
+        h = C.c_void_p()
+        rc = self.lib.tlm_batch_create(int(device), C.byref(h))
+        if rc != 0:
+            raise TlmError(f"tlm_batch_create failed ({rc}): "
+                           f"{self.lib.tlm_batch_last_error(None).decode()}")
+        self.h = h
+        self.members = {}
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise TlmError(f"{what} failed ({rc}): {self.lib.tlm_batch_last_error(self.h).decode()}")
+
+    def add(self, dev: TlmDevice, nsteps: int):
+        self._chk(self.lib.tlm_batch_add(self.h, dev.h, int(nsteps)), "tlm_batch_add")
+        self.members[dev.h.value] = dev
+
+    def remove(self, dev: TlmDevice):
+        self._chk(self.lib.tlm_batch_remove(self.h, dev.h), "tlm_batch_remove")
+        self.members.pop(dev.h.value, None)
+
+    def run(self, nticks: int):
+        self._chk(self.lib.tlm_batch_run(self.h, int(nticks)), "tlm_batch_run")
+
+    def sync(self):
+        self._chk(self.lib.tlm_batch_sync(self.h), "tlm_batch_sync")
+
+    def remaining(self):
+        """{TlmDevice: steps still to run} as of the ticks queued."""
+        n = self.lib.tlm_batch_members(self.h, None, None, 0)
+        hs = (C.c_void_p * max(n, 1))()
+        rem = np.zeros(max(n, 1), np.int32)
+        self.lib.tlm_batch_members(self.h, hs, ptr(rem, C.c_int32), n)
+        return {self.members[hs[i]]: int(rem[i]) for i in range(n)}
+
+    def set_tuning(self, planes=1, block_waves=4, grid_blocks=0, graph_ticks=32):
+        t = _native.BatchTuning(int(planes), int(block_waves), int(grid_blocks), int(graph_ticks))
+        self._chk(self.lib.tlm_batch_set_tuning(self.h, C.byref(t)), "tlm_batch_set_tuning")
+
+    def get_tuning(self) -> dict:
+        t = _native.BatchTuning()
+        self._chk(self.lib.tlm_batch_get_tuning(self.h, C.byref(t)), "tlm_batch_get_tuning")
+        return {f: getattr(t, f) for f, _ in t._fields_}
+
+    def time(self, nticks: int):
+        """(ms, cell updates) of nticks ticks timed with HIP events."""
+        ms = C.c_float(0)
+        cu = C.c_int64(0)
+        self._chk(self.lib.tlm_batch_time(self.h, int(nticks), C.byref(ms), C.byref(cu)),
+                  "tlm_batch_time")
+        return ms.value, cu.value
+
+    def autotune(self, ticks_per_candidate=4) -> dict:
+        t = _native.BatchTuning()
+        self._chk(self.lib.tlm_batch_autotune(self.h, int(ticks_per_candidate), C.byref(t)),
+                  "tlm_batch_autotune")
+        return {f: getattr(t, f) for f, _ in t._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.tlm_batch_destroy(self.h)
+            self.h = None
+            self.members = {}
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run_scenes_batched(scene_list, device=0, tuning=None, autotune=0, block=64):
+    """Run several scenes from step 0 in one batch; returns their SceneResults
+    (probes, ports, final state) in order."""
+    devs = [open_scene(sc, device) for sc in scene_list]
+    with TlmBatch(device) as b:
+        if tuning:
+            b.set_tuning(**tuning)
+        for d, sc in zip(devs, scene_list):
+            b.add(d, sc.total_steps)
+        left = max(sc.total_steps for sc in scene_list)
+        if autotune:
+            b.autotune(autotune)
+            b.sync()
+            left = max(b.remaining().values())
+        while left > 0:
+            m = min(block, left)
+            b.run(m)
+            left -= m
+        b.sync()
+        for d in devs:
+            b.remove(d)
+    out = [collect(d, sc, sc.total_steps) for d, sc in zip(devs, scene_list)]
+    for d in devs:
+        d.close()
+    return out

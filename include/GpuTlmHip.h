@@ -36,10 +36,20 @@ enum {
     TLM_ERR_NOMEM = -3,       /* device allocation failed (GpuTlm.cpp:664-668) */
     TLM_ERR_HIP = -4,         /* other HIP runtime error */
     TLM_ERR_STATE = -5,       /* call out of order (e.g. run before upload) */
-    TLM_ERR_UNSUPPORTED = -6  /* configuration this build does not handle */
+    TLM_ERR_UNSUPPORTED = -6, /* configuration this build does not handle */
+    TLM_ERR_RANGE = -7        /* 16-bit storage: a value outgrew its planned slab scale */
 };
 
-/* Link-pulse storage type. Arithmetic is always FP32. */
+/* Link-pulse storage type. Arithmetic is always FP32.
+ * TLM_PULSE_F16 / TLM_PULSE_BF16 (no counterpart in the reference): each
+ * z-slab holds its pulses either as FP32 (slabs within fp32_halo slabs of a
+ * soft source, where the strong reactive near field makes 16-bit rounding
+ * costly) or as IEEE binary16 / bfloat16 codes times a per-slab power of two
+ * 2^E. E is planned one step ahead from the largest value each slab sent the
+ * step before (one step grows the max norm at most 5x and moves a pulse one
+ * slab), so no value can overflow; tlm_sync reports TLM_ERR_RANGE if one
+ * ever did. A value is rounded (to nearest even) once, where it is stored.
+ * Stubs, probe records, DFT and energy accumulators keep FP32 / FP64. */
 enum { TLM_PULSE_F32 = 0, TLM_PULSE_F16 = 1, TLM_PULSE_BF16 = 2 };
 
 typedef struct tlm_grid_desc {
@@ -48,10 +58,12 @@ typedef struct tlm_grid_desc {
     float rho;           /* outer boundary reflection, TlmConfig::boundaryRho (TlmSolver.h:119) */
     int32_t pulse_type;  /* TLM_PULSE_* */
     int32_t device;      /* HIP device ordinal */
-    int32_t reserved;    /* must be 0 */
+    int32_t fp32_halo;   /* 16-bit formats: FP32 z-slabs on each side of a source slab;
+                            0 = default (8), < 0 = none */
 } tlm_grid_desc;
 
 typedef struct tlm_ctx tlm_ctx;
+typedef struct tlm_batch tlm_batch;
 
 /* Number of visible HIP devices (0 without a GPU). */
 int tlm_device_count(void);
@@ -133,8 +145,22 @@ int tlm_set_frames(tlm_ctx *ctx, int every, int max_frames);
  * scatter/connect (TlmSolver::step, TlmSolver.cpp:485-513). Asynchronous. */
 int tlm_run(tlm_ctx *ctx, int step0, int nsteps);
 
-/* Waits for the queued work; reports asynchronous HIP errors. */
+/* Waits for the queued work; reports asynchronous HIP errors and, for the
+ * 16-bit formats, TLM_ERR_RANGE. A 16-bit context runs contiguous step
+ * blocks (each tlm_run starts where the previous one ended). */
 int tlm_sync(tlm_ctx *ctx);
+
+/* 16-bit formats (no counterpart in the reference): per z-slab, the decode
+ * scale 2^E of the current step, the largest |value| the slab's cells sent
+ * in the last step, and the slab's storage format (TLM_PULSE_*). Any pointer
+ * may be NULL. Returns nz, or 0 for FP32 contexts and before the first run. */
+int tlm_read_scales(tlm_ctx *ctx, float *scale, float *smax, int32_t *fmt);
+
+/* Rounds in[i] * inv_scale to binary16 (fmt TLM_PULSE_F16) or bfloat16
+ * (TLM_PULSE_BF16) with the device conversion the step kernel uses, for
+ * checking it against the IEEE round-to-nearest-even definition. */
+int tlm_device_encode(int device, int fmt, const float *in, float inv_scale, uint16_t *out,
+                      int64_t n);
 
 /* Current pulses back in reference AoS order (for snapshot parity; the GL
  * path never read V back). Vs_aos3 may be NULL. Synchronous. */
@@ -231,6 +257,56 @@ enum { TLM_WF_CW_RAMPED = 0, TLM_WF_GAUSS_SINE = 1, TLM_WF_GAUSS_PULSE = 2, TLM_
  * step0 .. step0+nsteps-1 (out[n - step0]). Any output pointer may be NULL. */
 void tlm_schedule_tables(int waveform, float f0, float dt, int step0, int nsteps, float gd_width,
                          float gd_delay, float *sv, float *cph, float *sph);
+
+/* ---------------------------------------------------------------------
+ * Batched execution (no counterpart in the reference, whose GPU path runs
+ * one grid per OpenGL context): many independent contexts advanced by the
+ * same launches, one tick = one step of every active member. Members are
+ * ordinary contexts set up with the calls above (probes and ports only: no
+ * Js, Huygens, plane or frame monitors; no energy records while batched).
+ * A member gives the same bits alone and in a batch. While a context is a
+ * member, tlm_run and the tlm_set_* / tlm_upload calls on it are refused;
+ * its readers work after tlm_batch_sync.
+ * ------------------------------------------------------------------- */
+typedef struct tlm_batch_tuning {
+    int32_t planes;      /* z slabs per wave task (cells per lane); 0 = all of nz */
+    int32_t block_waves; /* wavefronts (64-cell strips) per workgroup, 1..4; 0 = 4 */
+    int32_t grid_blocks; /* > 0: persistent grid of that many workgroups; 0 = one per group */
+    int32_t graph_ticks; /* ticks per captured HIP graph (default 32); 0 = plain launches */
+} tlm_batch_tuning;
+
+int tlm_batch_create(int device, tlm_batch **out);
+void tlm_batch_destroy(tlm_batch *b);
+const char *tlm_batch_last_error(const tlm_batch *b);
+
+/* ctx joins at its next step and runs nsteps more steps (within its
+ * schedule), starting with the next tick. */
+int tlm_batch_add(tlm_batch *b, tlm_ctx *ctx, int nsteps);
+/* Waits for queued ticks; ctx leaves the batch and is a plain context again. */
+int tlm_batch_remove(tlm_batch *b, tlm_ctx *ctx);
+/* Queues nticks ticks (asynchronous); finished members idle. */
+int tlm_batch_run(tlm_batch *b, int nticks);
+/* Waits for the queued ticks and updates every member (step counters,
+ * running probe DFT); reports TLM_ERR_RANGE for a 16-bit member. */
+int tlm_batch_sync(tlm_batch *b);
+/* Members and their remaining steps (as of the ticks queued); returns the
+ * member count. */
+int tlm_batch_members(tlm_batch *b, tlm_ctx **members, int32_t *remaining, int cap);
+int tlm_batch_set_tuning(tlm_batch *b, const tlm_batch_tuning *t);
+int tlm_batch_get_tuning(const tlm_batch *b, tlm_batch_tuning *t);
+/* Runs nticks ticks timed with HIP events; cell_updates = cells x steps of
+ * the members active in them. */
+int tlm_batch_time(tlm_batch *b, int nticks, float *ms, int64_t *cell_updates);
+/* Online autotuner: times real ticks of the batch under each candidate
+ * launch shape (slabs per wave task, waves per workgroup, grid) and keeps
+ * the fastest per cell update; results do not depend on the shape. */
+int tlm_batch_autotune(tlm_batch *b, int ticks_per_candidate, tlm_batch_tuning *best);
+
+/* Free and total device memory (admission control of a serving loop). */
+int tlm_device_memory(int device, size_t *free_bytes, size_t *total_bytes);
+/* Device bytes a context of this shape allocates (pulses, stubs if any
+ * dielectric, cell codes and per-step work buffers; monitors excluded). */
+int64_t tlm_estimate_bytes(const tlm_grid_desc *desc, int has_stubs);
 
 #ifdef __cplusplus
 }

@@ -27,7 +27,8 @@ class _Cfg(C.Structure):
         ("total_steps", C.c_int32), ("settle_steps", C.c_int32),
         ("rho", C.c_float), ("huy_offset", C.c_int32), ("record_every", C.c_int32),
         ("gd_width", C.c_float), ("gd_delay", C.c_float),
-        ("threads", C.c_int32), ("fp64", C.c_int32),
+        ("threads", C.c_int32), ("fp64", C.c_int32), ("pulse_format", C.c_int32),
+        ("fp32_halo", C.c_int32),
     ]
 
 
@@ -49,6 +50,8 @@ class OracleConfig:
     gd_delay: float = 60.0
     threads: int = 0
     fp64: bool = False
+    pulse_format: int = 0  # 0 FP32, 1 FP16, 2 BF16 storage (float oracle only)
+    fp32_halo: int = 8     # 16-bit formats: slabs within this distance of a source stay FP32
 
 
 _lib = None
@@ -113,6 +116,12 @@ def load_library():
         "ot_num_frames": (C.c_int, [P]),
         "ot_frame": (C.c_int, [P, C.c_int, dp, dp]),
         "ot_scatter_cell": (None, [dp, dp, C.c_double, C.c_double, C.c_int, C.c_int, dp]),
+        "ot_slab_exponents": (C.c_int, [P, i32p, fp]),
+        "ot_f16_rne": (C.c_uint16, [C.c_float]),
+        "ot_f16_to_f32": (C.c_float, [C.c_uint16]),
+        "ot_bf16_rne": (C.c_uint16, [C.c_float]),
+        "ot_plan_exponent": (C.c_int, [C.c_float]),
+        "ot_encode": (None, [C.c_int, fp, C.c_float, C.POINTER(C.c_uint16), C.c_int64]),
         "ot_scatter_unitarity_error": (C.c_float, []),
         "ot_diel_scatter_unitarity_error": (C.c_float, [C.c_float]),
     }
@@ -144,7 +153,8 @@ class OracleTlm:
         assert epsr.size == sigma.size
         c = _Cfg(cfg.nx, cfg.ny, cfg.nz, cfg.dl, *cfg.origin, cfg.f0, cfg.waveform,
                  cfg.total_steps, cfg.settle_steps, cfg.rho, cfg.huy_offset,
-                 cfg.record_every, cfg.gd_width, cfg.gd_delay, cfg.threads, int(cfg.fp64))
+                 cfg.record_every, cfg.gd_width, cfg.gd_delay, cfg.threads, int(cfg.fp64),
+                 int(cfg.pulse_format), int(cfg.fp32_halo))
         self.h = self.lib.ot_create(C.byref(c), _ptr(mat, C.c_uint8), _ptr(epsr, C.c_float),
                                     _ptr(sigma, C.c_float), int(epsr.size))
 
@@ -333,6 +343,13 @@ class OracleTlm:
                 planes[i] = pl.reshape(n2.value, n1.value)
         return steps, js, planes
 
+    def slab_exponents(self):
+        """(E_n per slab, M_{n-1} per slab) of the 16-bit storage mode."""
+        E = np.zeros(self.cfg.nz, np.int32)
+        M = np.zeros(self.cfg.nz, np.float32)
+        n = self.lib.ot_slab_exponents(self.h, _ptr(E, C.c_int32), _ptr(M, C.c_float))
+        return (E, M) if n else (None, None)
+
     def frames(self):
         out = []
         nf = self.lib.ot_num_faces(self.h)
@@ -363,3 +380,17 @@ def scatter_unitarity_error() -> float:
 
 def diel_scatter_unitarity_error(epsr: float) -> float:
     return load_library().ot_diel_scatter_unitarity_error(epsr)
+
+
+def encode_pulses(fmt: int, values, inv_scale: float = 1.0) -> np.ndarray:
+    """16-bit codes of values * inv_scale (fmt 1 binary16, 2 bfloat16), rounded
+    to nearest even by the oracle's software conversion."""
+    lib = load_library()
+    v = np.ascontiguousarray(values, np.float32)
+    out = np.zeros(v.size, np.uint16)
+    lib.ot_encode(int(fmt), _ptr(v, C.c_float), C.c_float(inv_scale), _ptr(out, C.c_uint16), v.size)
+    return out.reshape(v.shape)
+
+
+def plan_exponent(bound: float) -> int:
+    return int(load_library().ot_plan_exponent(C.c_float(bound)))

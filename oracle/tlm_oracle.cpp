@@ -255,6 +255,124 @@ float waveform(int type, float f0, float dt, int n, float gdWidth, float gdDelay
 }
 
 //---------------------------------------------------------------------------
+//---------------------------------------------------------------------------
+// Reduced-precision pulse storage (FP16 / BF16 with per-slab power-of-two
+// scales). The reference has no such mode: this restates the scheme the HIP
+// path implements so that its 16-bit runs can be checked bit for bit, while
+// the double instantiation measures the physics error of the scheme.
+//
+//   * A stored pulse of z-slab k at step n is code * 2^E_n[k], code an IEEE
+//     binary16 (or bfloat16) number rounded to nearest even.
+//   * The scatter arithmetic stays FP32 on decoded values; a value is
+//     rounded to 16 bits once, where it is stored (injection, delivery by
+//     connect, kept PEC values).
+//   * E_{n+1} is chosen before step n from the per-slab maximum M_{n-1} of
+//     the values every cell sent at step n-1 (|out| of non-PEC cells and their
+//     new stubs, |in| of PEC cells) and from the injection bound Q[k] =
+//     sum of 0.5*|amp| over the slab's soft sources. One step at most
+//     multiplies the max norm by 5 (|V_j| <= 2 max, |ZI| <= 2 max, |in|) and
+//     a pulse moves at most one slab, so
+//       bound = (rp*6) * (rp*max_{|d|<=2} M[k+d] + max_{|d|<=1} Q[k+d]*|sv_n|)
+//               + Q[k]*|sv_{n+1}|,     rp = max(1, |rho|)
+//     bounds every value stored into slab k at step n+1, and E is picked so
+//     that bound * 2^-E < 2^15 (binary16 holds up to 65504).
+//---------------------------------------------------------------------------
+enum { PF_F32 = 0, PF_F16 = 1, PF_BF16 = 2 };
+
+inline uint32_t fbits(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+inline float bitsf(uint32_t u)
+{
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+// 2^e for -126 <= e <= 127
+inline float pow2f(int e) { return bitsf((uint32_t)(e + 127) << 23); }
+
+// float -> binary16, round to nearest even, subnormals kept (IEEE 754).
+inline uint16_t f16_rne(float f)
+{
+    uint32_t x = fbits(f);
+    const uint32_t sign = (x >> 16) & 0x8000u;
+    x &= 0x7fffffffu;
+    if (x > 0x7f800000u)
+        return (uint16_t)(sign | 0x7e00u); // NaN
+    if (x >= 0x477ff000u)
+        return (uint16_t)(sign | 0x7c00u); // rounds to infinity (>= 65520)
+    if (x >= 0x38800000u)
+    {
+        // normal binary16: drop 13 mantissa bits
+        const uint32_t mant = x & 0x7fffffu;
+        uint32_t h = (((x >> 23) - 112u) << 10) | (mant >> 13);
+        const uint32_t rem = mant & 0x1fffu;
+        if (rem > 0x1000u || (rem == 0x1000u && (h & 1u)))
+            ++h; // a carry into the exponent is the correct result
+        return (uint16_t)(sign | h);
+    }
+    if (x < 0x33000000u)
+        return (uint16_t)sign; // below 2^-25: rounds to zero
+    // subnormal binary16, quantum 2^-24
+    const uint32_t e = x >> 23;
+    const uint32_t mant = (x & 0x7fffffu) | 0x800000u;
+    const uint32_t shift = 126u - e; // 14..24
+    uint32_t h = mant >> shift;
+    const uint32_t rem = mant & ((1u << shift) - 1u), half = 1u << (shift - 1u);
+    if (rem > half || (rem == half && (h & 1u)))
+        ++h;
+    return (uint16_t)(sign | h);
+}
+inline float f16_to_f32(uint16_t h)
+{
+    const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+    const uint32_t e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+    if (e == 0)
+    {
+        const float v = (float)m * pow2f(-24);
+        return sign ? -v : v;
+    }
+    if (e == 31)
+        return bitsf(sign | 0x7f800000u | (m << 13));
+    return bitsf(sign | ((e + 112u) << 23) | (m << 13));
+}
+// float -> bfloat16, round to nearest even (finite inputs)
+inline uint16_t bf16_rne(float f)
+{
+    uint32_t u = fbits(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+inline float bf16_to_f32(uint16_t h) { return bitsf((uint32_t)h << 16); }
+
+// Slab exponent for a bound: bound * 2^-E < 2^15, E in [-100, 100].
+inline int plan_exponent(float bound)
+{
+    if (!(bound > 0.0f))
+        return 0;
+    const int be = (int)((fbits(bound) >> 23) & 0xffu);
+    if (be == 0)
+        return -100;
+    return std::min(100, std::max(-100, be - 126 - 15));
+}
+inline float plan_bound(float rp, float Mn, float Qn, float svn, float Qk, float svn1)
+{
+    return (rp * 6.0f) * ((rp * Mn) + (Qn * svn)) + (Qk * svn1);
+}
+// decode(encode(v)): the value a stored 16-bit pulse represents
+inline float quant(int fmt, float v, int E)
+{
+    if (fmt == PF_F32)
+        return v;
+    const float s = v * pow2f(-E);
+    const float d = (fmt == PF_F16) ? f16_to_f32(f16_rne(s)) : bf16_to_f32(bf16_rne(s));
+    return d * pow2f(E);
+}
+template <class T> inline T quant(int, T v, int) { return v; }
+
 struct SolverBase
 {
     virtual ~SolverBase() {}
@@ -296,6 +414,14 @@ template <class T> struct Solver : SolverBase
     int curStep = 0;
     float dt = 0.0f;
     std::unique_ptr<WorkPool> pool;
+    // reduced-precision storage (float instantiation only)
+    int pfmt = PF_F32;
+    bool qReady = false;                 // state encoded with the slab exponents
+    std::vector<int> Ecur, Enext;        // per slab: E_n, E_{n+1}
+    std::vector<float> Mreg, Mnew, Qinj; // per slab: M_{n-1}, M_n, injection bound
+    int fp32Halo = -1;                   // slabs within this distance of a source slab stay FP32
+    std::vector<uint8_t> hot;            // per slab: 1 = FP32 storage
+    int slabFmt(size_t k) const { return hot[k] ? PF_F32 : pfmt; }
 
     size_t ncell() const { return (size_t)nx * ny * nz; }
     size_t cellIndex(int i, int j, int k) const { return ((size_t)k * ny + j) * nx + i; }
@@ -507,15 +633,27 @@ template <class T> struct Solver : SolverBase
     {
         float svf = waveformValue(n);
         T sv = T(svf);
+        if (pfmt != PF_F32)
+        {
+            if (!qReady)
+                encodeState(svf);
+            planSlabs(n, svf);
+        }
+        const size_t slab = (size_t)nx * ny;
         auto inject = [&](int64_t cell, int polAxis, float amp) {
             T q = T(0.5) * T(amp) * sv;
             T *vp = &V[(size_t)cell * 12];
+            const size_t k = (size_t)cell / slab;
+            const int E = pfmt ? Ecur[k] : 0;
+            const int fm = pfmt ? slabFmt(k) : PF_F32;
             for (int axis = 0; axis < 3; ++axis)
             {
                 if (axis == polAxis)
                     continue;
-                vp[portOf(axis, 0, polAxis)] += q;
-                vp[portOf(axis, 1, polAxis)] += q;
+                T &a = vp[portOf(axis, 0, polAxis)];
+                a = quant(fm, a + q, E);
+                T &b = vp[portOf(axis, 1, polAxis)];
+                b = quant(fm, b + q, E);
             }
         };
         for (const auto &s : sources)
@@ -526,6 +664,97 @@ template <class T> struct Solver : SolverBase
         monitors(n);
         scatter();
         connect();
+        if (pfmt != PF_F32)
+        {
+            // every slot is stored with its slab's E_{n+1}
+            pool->run(pool->size(), [&](int c) {
+                int k0 = (int)((long long)nz * c / pool->size());
+                int k1 = (int)((long long)nz * (c + 1) / pool->size());
+                for (int k = k0; k < k1; ++k)
+                {
+                    if (hot[k])
+                        continue;
+                    T *v = &V[(size_t)k * slab * 12];
+                    for (size_t i = 0; i < slab * 12; ++i)
+                        v[i] = quant(pfmt, v[i], Enext[k]);
+                }
+            });
+            Ecur.swap(Enext);
+            Mreg.swap(Mnew);
+        }
+    }
+
+    // Injection bound per slab: sum of 0.5*|amp| over the soft sources, then
+    // the port cells, in list order.
+    // Also marks the FP32 slabs: within fp32Halo slabs of a source slab
+    // (the strong reactive near field of a soft source is where 16-bit
+    // rounding costs the most accuracy).
+    void injectionBound()
+    {
+        Qinj.assign(nz, 0.0f);
+        hot.assign(nz, 0);
+        const size_t slab = (size_t)nx * ny;
+        auto mark = [&](int64_t c, float amp) {
+            const int k = (int)((size_t)c / slab);
+            Qinj[k] = Qinj[k] + 0.5f * std::fabs(amp);
+            if (fp32Halo >= 0)
+                for (int d = -fp32Halo; d <= fp32Halo; ++d)
+                    if (k + d >= 0 && k + d < nz)
+                        hot[k + d] = 1;
+        };
+        for (const auto &s : sources)
+            mark(s.cell, s.amp);
+        for (const auto &p : ports)
+            for (int64_t c : p.cells)
+                mark(c, p.amp);
+    }
+
+    // First step after set-up / set_state: slab maxima of the FP32 state,
+    // E = exponent of (M + Q*|sv|), state rounded to 16 bits.
+    void encodeState(float svf)
+    {
+        injectionBound();
+        const size_t slab = (size_t)nx * ny;
+        Mreg.assign(nz, 0.0f);
+        Mnew.assign(nz, 0.0f);
+        Ecur.assign(nz, 0);
+        Enext.assign(nz, 0);
+        for (int k = 0; k < nz; ++k)
+        {
+            float m = 0.0f;
+            for (size_t i = (size_t)k * slab * 12; i < (size_t)(k + 1) * slab * 12; ++i)
+                m = std::max(m, (float)std::fabs(V[i]));
+            if (!Vs.empty())
+                for (size_t i = (size_t)k * slab * 3; i < (size_t)(k + 1) * slab * 3; ++i)
+                    m = std::max(m, (float)std::fabs(Vs[i]));
+            Mreg[k] = m;
+            Ecur[k] = plan_exponent(m + Qinj[k] * std::fabs(svf));
+            if (hot[k])
+                continue;
+            T *v = &V[(size_t)k * slab * 12];
+            for (size_t i = 0; i < slab * 12; ++i)
+                v[i] = quant(pfmt, v[i], Ecur[k]);
+        }
+        qReady = true;
+    }
+
+    // E_{n+1} from M_{n-1} and the injection bound (see the scheme above).
+    void planSlabs(int n, float svf)
+    {
+        const float rp = std::max(1.0f, std::fabs(rho));
+        const float svn = std::fabs(svf);
+        const float svn1 = (n + 1 < totalSteps) ? std::fabs(waveformValue(n + 1)) : 0.0f;
+        for (int k = 0; k < nz; ++k)
+        {
+            float Mn = 0.0f, Qn = 0.0f;
+            for (int d = -2; d <= 2; ++d)
+                if (k + d >= 0 && k + d < nz)
+                    Mn = std::max(Mn, Mreg[k + d]);
+            for (int d = -1; d <= 1; ++d)
+                if (k + d >= 0 && k + d < nz)
+                    Qn = std::max(Qn, Qinj[k + d]);
+            Enext[k] = plan_exponent(plan_bound(rp, Mn, Qn, svn, Qinj[k], svn1));
+        }
     }
 
     // TlmSolver.cpp:516-548
@@ -540,22 +769,36 @@ template <class T> struct Solver : SolverBase
             for (int k = k0; k < k1; ++k)
             {
                 size_t cell = slab * k;
+                T mk = T(0); // reduced storage: largest value a cell of the slab sends
                 for (size_t q = 0; q < slab; ++q, ++cell)
                 {
                     uint8_t mm = mat[cell];
                     if (mm == 1)
+                    {
+                        if (pfmt)
+                            for (int p = 0; p < 12; ++p)
+                                mk = std::max(mk, (T)std::fabs(V[cell * 12 + p]));
                         continue;
+                    }
                     T vin[12], vout[12];
                     std::memcpy(vin, &V[cell * 12], sizeof(vin));
                     if (mm >= 2 && haveStubs)
                     {
                         int id = mm - 2;
                         ScatterCellDiel(vin, &Vs[cell * 3], yhat[id], ghat[id], vout);
+                        if (pfmt)
+                            for (int p = 0; p < 3; ++p)
+                                mk = std::max(mk, (T)std::fabs(Vs[cell * 3 + p]));
                     }
                     else
                         ScatterCell(vin, vout);
+                    if (pfmt)
+                        for (int p = 0; p < 12; ++p)
+                            mk = std::max(mk, (T)std::fabs(vout[p]));
                     std::memcpy(&V[cell * 12], vout, sizeof(vout));
                 }
+                if (pfmt)
+                    Mnew[k] = (float)mk;
             }
         });
     }
@@ -850,6 +1093,8 @@ ot_handle *ot_create(const ot_config *cfg, const uint8_t *mat, const float *epsr
         s->recordEvery = cfg->record_every;
         s->gdWidth = cfg->gd_width;
         s->gdDelay = cfg->gd_delay;
+        s->pfmt = cfg->fp64 ? PF_F32 : cfg->pulse_format;
+        s->fp32Halo = cfg->fp32_halo;
         int nt = cfg->threads > 0 ? cfg->threads : ot_default_threads();
         s->pool.reset(new WorkPool(nt));
         s->setup(mat, epsr, sigma, nmat);
@@ -921,6 +1166,7 @@ void ot_set_state(ot_handle *h, const double *V, const double *Vs)
         if (Vs)
             for (size_t i = 0; i < s.Vs.size(); ++i)
                 s.Vs[i] = (T)Vs[i];
+        s.qReady = false;
         return 0;
     });
 }
@@ -958,6 +1204,30 @@ void ot_finalize(ot_handle *h)
         s.finalizeDft();
         return 0;
     });
+}
+int ot_slab_exponents(ot_handle *h, int32_t *E, float *M)
+{
+    return dispatch(h, [&](auto &s) {
+        if (!s.qReady)
+            return 0;
+        for (int k = 0; k < s.nz; ++k)
+        {
+            if (E)
+                E[k] = s.Ecur[k];
+            if (M)
+                M[k] = s.Mreg[k];
+        }
+        return s.nz;
+    });
+}
+uint16_t ot_f16_rne(float f) { return f16_rne(f); }
+float ot_f16_to_f32(uint16_t h) { return f16_to_f32(h); }
+uint16_t ot_bf16_rne(float f) { return bf16_rne(f); }
+int ot_plan_exponent(float bound) { return plan_exponent(bound); }
+void ot_encode(int fmt, const float *in, float inv_scale, uint16_t *out, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i)
+        out[i] = (fmt == PF_F16) ? f16_rne(in[i] * inv_scale) : bf16_rne(in[i] * inv_scale);
 }
 int ot_cur_step(ot_handle *h)
 {

@@ -27,6 +27,10 @@ typedef struct ot_config {
     float gd_width, gd_delay;    /* Gaussian-derivative width / delay, in steps */
     int32_t threads;             /* <= 0: min(16, hw-1) like TlmSolver.cpp:247-249 */
     int32_t fp64;                /* 0: float (parity oracle), 1: double (precision oracle) */
+    int32_t pulse_format;        /* float only: 0 FP32, 1 FP16, 2 BF16 pulse storage with
+                                    per-slab power-of-two scales (no reference counterpart) */
+    int32_t fp32_halo;           /* 16-bit formats: z-slabs within this distance of a source
+                                    slab keep FP32 storage; < 0: none */
 } ot_config;
 
 typedef struct ot_handle ot_handle;
@@ -52,6 +56,12 @@ void ot_run_steps(ot_handle *h, int count);
 void ot_scatter_connect(ot_handle *h);
 void ot_finalize(ot_handle *h);
 
+int ot_slab_exponents(ot_handle *h, int32_t *E, float *M);
+uint16_t ot_f16_rne(float f);
+float ot_f16_to_f32(uint16_t h);
+uint16_t ot_bf16_rne(float f);
+int ot_plan_exponent(float bound);
+void ot_encode(int fmt, const float *in, float inv_scale, uint16_t *out, int64_t n);
 int ot_cur_step(ot_handle *h);
 int ot_has_stubs(ot_handle *h);
 float ot_dt(ot_handle *h);

@@ -11,13 +11,19 @@ from emsolver_amd.tlm import SceneResult
 from oracle.oracle_tlm import OracleConfig, OracleTlm
 
 
+PULSE_FORMATS = {"f32": 0, "f16": 1, "bf16": 2}
+
+
 def oracle_open(scene, fp64=False, threads=0) -> OracleTlm:
+    """The oracle for a scene: FP32 (parity), FP64 (precision reference) or,
+    for a 16-bit scene, the restatement of its 16-bit storage scheme."""
     cfg = OracleConfig(scene.nx, scene.ny, scene.nz, dl=scene.dl, origin=scene.origin,
                        f0=scene.f0, waveform=scene.waveform, total_steps=scene.total_steps,
                        settle_steps=scene.settle_steps, rho=scene.rho,
                        huy_offset=scene.huy_offset, record_every=scene.record_every,
                        gd_width=scene.gd_width, gd_delay=scene.gd_delay, threads=threads,
-                       fp64=fp64)
+                       fp64=fp64, pulse_format=0 if fp64 else PULSE_FORMATS[scene.pulse],
+                       fp32_halo=scene.fp32_halo)
     o = OracleTlm(cfg, scene.mat, scene.epsr, scene.sigma)
     if scene.plane_axis >= 0:
         o.set_field_plane(scene.plane_axis, scene.plane_index)

@@ -50,7 +50,7 @@ def test_argument_checks_before_device():
     assert _create(0, 4, 4)[0] == -1
     assert _create(4, 4, 4, dl=0.0)[0] == -1
     rc, _, msg = _create(4, 4, 4, pulse=7)
-    assert rc == -6 and "FP32" in msg
+    assert rc == -1 and "storage type" in msg
     rc, _, msg = _create(1100, 1100, 1000)
     assert rc == -6 and "32-bit" in msg
     lib = _native.load()

@@ -102,7 +102,7 @@ def test_wide_rows_two_waves():
 
 
 def test_free_space_64_full_run_tolerances():
-    """North-star tolerances on the 64^3 free-space case: probe max rel err
+    """The FP32 tolerances on the 64^3 free-space case: probe max rel err
     <= 1e-5 and L2-normalised snapshot err <= 1e-6 (bit-exact in practice)."""
     sc = scenes.free_space_cube(n=64, steps=1000)
     g = tlm.run_scene(sc)

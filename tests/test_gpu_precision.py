@@ -1,0 +1,128 @@
+"""GPU parity of the 16-bit pulse storage (FP16 / BF16 codes times per-slab
+powers of two, FP32 slabs near soft sources) against the oracle's
+restatement of the same scheme.
+
+Bar: bit-exact pulses, stubs, probes, ports, Js and probe DFT, equal slab
+scales; FP64 energy within 1e-12 relative. The device rounding itself is
+first checked against the IEEE round-to-nearest-even definition.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from emsolver_amd import scenes, tlm
+from oracle import oracle_tlm as O
+from tests import helpers
+from tests.test_gpu_parity import TUNINGS, _random_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def _rounding_sample():
+    rng = np.random.default_rng(0)
+    n = 300_000
+    e = rng.integers(-40, 17, n)
+    x = ((rng.random(n) + 1.0) * 2.0 ** e * rng.choice([-1, 1], n)).astype(np.float32)
+    h = rng.integers(0, 0x7bff, 50_000).astype(np.uint16)
+    lo = h.view(np.float16).astype(np.float64)
+    hi = (h + 1).astype(np.uint16).view(np.float16).astype(np.float64)
+    ties = ((lo + hi) / 2).astype(np.float32)
+    u = x.view(np.uint32)
+    bties = ((u[:20000] & 0xFFFF0000) | 0x8000).view(np.float32)
+    return np.concatenate([x, ties, -ties, bties, np.float32([0.0, -0.0, 65504.0, 2.0 ** -25])])
+
+
+@pytest.mark.parametrize("fmt,code", [("f16", 1), ("bf16", 2)])
+def test_device_rounding_is_ieee_nearest_even(fmt, code):
+    x = _rounding_sample()
+    for inv in (1.0, 2.0 ** -20, 2.0 ** 17):
+        got = tlm.device_encode(fmt, x, inv)
+        want = O.encode_pulses(code, x, inv)
+        bad = np.flatnonzero(got != want)
+        assert bad.size == 0, (fmt, inv, x[bad[:4]], got[bad[:4]], want[bad[:4]])
+
+
+def _check16(sc, steps=None, tuning=None):
+    print(f"[parity16] {sc.name} {sc.pulse} halo={sc.fp32_halo} rho={sc.rho} "
+          f"steps={steps or sc.total_steps} tuning={tuning}", flush=True)
+    steps = sc.total_steps if steps is None else steps
+    with tlm.open_scene(sc) as dev:
+        if tuning:
+            dev.set_tuning(**tuning)
+        n = 0
+        while n < steps:
+            m = min(37, steps - n)  # several blocks: the scales carry across tlm_run calls
+            dev.run(n, m)
+            n += m
+        dev.sync()
+        g = tlm.collect(dev, sc, steps)
+        scales = dev.read_scales()
+    o = helpers.oracle_open(sc)
+    o.run_steps(steps)
+    o.finalize()
+    r = helpers.oracle_collect(o, sc)
+    E, _ = o.slab_exponents()
+    o.close()
+    helpers.compare_results(g, r, sc)
+    assert scales is not None
+    assert np.array_equal(scales[0], (2.0 ** E.astype(np.float64)).astype(np.float32))
+    hot = scales[2] == 0
+    assert (hot.any() if sc.fp32_halo >= 0 and (sc.sources or sc.ports) else not hot.any())
+    return g, r
+
+
+@pytest.mark.parametrize("pulse", ["f16", "bf16"])
+@pytest.mark.parametrize("halo", [-1, 2])
+def test_boxes_bitexact(pulse, halo):
+    sc = scenes.dielectric_boxes(n=36, nboxes=5, steps=120, nprobes=6, nfreq=4, seed=4)
+    _check16(dataclasses.replace(sc, pulse=pulse, fp32_halo=halo))
+
+
+@pytest.mark.parametrize("shape", [(13, 7, 9), (33, 5, 16), (64, 3, 17)])
+@pytest.mark.parametrize("rho", [0.0, -1.0, 0.37])
+def test_random_materials_bitexact(shape, rho):
+    """PEC cells, stubs, random initial state (stored at the first step with
+    scales from its slab maxima), several sources in one cell."""
+    sc = _random_scene(*shape, seed=sum(shape) + 3, rho=rho, steps=30)
+    _check16(dataclasses.replace(sc, pulse="f16", fp32_halo=1))
+    _check16(dataclasses.replace(sc, pulse="bf16", fp32_halo=-1))
+
+
+def test_dipole_port_bitexact():
+    sc = scenes.dipole_scene(24, 22, 40, length_frac=0.5, steps=150, background_boxes=3, seed=2)
+    sc.probes = np.array([sc.cell(3, 4, 5), sc.cell(20, 11, 30), sc.cell(12, 11, 20)], np.int64)
+    g, _ = _check16(dataclasses.replace(sc, pulse="f16", fp32_halo=4))
+    assert np.abs(g.ports).max() > 0
+
+
+@pytest.mark.parametrize("tuning", TUNINGS[1:4],
+                         ids=lambda t: "z{planes}_w{block_waves}_g{grid_blocks}".format(**t))
+def test_tunings_bitexact(tuning):
+    sc = _random_scene(300, 5, 9, seed=21, rho=0.37, steps=12)
+    _check16(dataclasses.replace(sc, pulse="f16", fp32_halo=1), tuning=tuning)
+
+
+def test_read_state_before_first_step():
+    sc = _random_scene(10, 9, 8, seed=3, steps=5)
+    sc = dataclasses.replace(sc, pulse="f16")
+    with tlm.open_scene(sc) as dev:
+        V, Vs = dev.read_state()
+        assert np.array_equal(V, sc.V0)
+        assert np.array_equal(Vs, sc.Vs0)
+        assert dev.read_scales() is None
+
+
+def test_error_against_fp64_dipole():
+    """FP16 with the FP32 halo against the FP64 oracle on a dipole with a
+    dielectric background (max relative error per probe and port)."""
+    sc = scenes.dipole_scene(28, 28, 40, length_frac=0.5, steps=600, background_boxes=3, seed=2)
+    rng = np.random.default_rng(5)
+    sc.probes = np.array([sc.cell(int(rng.integers(2, 26)), int(rng.integers(2, 26)),
+                                  int(rng.integers(2, 38))) for _ in range(4)], np.int64)
+    g = tlm.run_scene(dataclasses.replace(sc, pulse="f16", fp32_halo=8))
+    r64 = helpers.oracle_run(sc, fp64=True)
+    pe = np.abs(g.probes - r64.probes).max(axis=(0, 2)) / np.abs(r64.probes).max(axis=(0, 2))
+    ve = np.abs(g.ports - r64.ports).max(axis=0) / np.abs(r64.ports).max(axis=0)
+    print(f"probe max rel err {np.round(pe, 5)}  port V,I {np.round(ve.ravel(), 6)}")
+    assert pe.max() <= 2.5e-3 and ve.max() <= 1e-4
