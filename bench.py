@@ -112,6 +112,109 @@ def gpu_run(scene, warmup, steps, tag):
     return out
 
 
+def batched_run(count=256, sample_every=4):
+    """256 independent dipole grids (axes log-uniform in [32,128], steps
+    U[500,2000]) in one batch: autotuned launch shape, graph-captured blocks
+    of ticks, run to completion. Baseline: the same grids one at a time as
+    plain contexts (every `sample_every`-th grid, timed alone)."""
+    from emsolver_amd import scenes, tlm
+    scs = scenes.batch_dipoles(count=count, seed=3)
+    work = sum(sc.ncell * sc.total_steps for sc in scs)
+    t = time.perf_counter()
+    devs = [tlm.open_scene(sc) for sc in scs]
+    log(f"[bench] batched_{count}: {sum(sc.ncell for sc in scs) / 1e6:.1f} M cells, "
+        f"{work / 1e9:.1f} G cell-updates, set-up {time.perf_counter() - t:.1f}s")
+    with tlm.TlmBatch() as b:
+        for d, sc in zip(devs, scs):
+            b.add(d, sc.total_steps)
+        t0 = time.perf_counter()
+        tuning = b.autotune(4)
+        b.sync()
+        left = max(b.remaining().values())
+        while left > 0:
+            m = min(256, left)
+            b.run(m)
+            left -= m
+        b.sync()
+        wall = time.perf_counter() - t0
+        probe_absmax = 0.0
+        for d, sc in zip(devs, scs):
+            b.remove(d)
+        for d, sc in zip(devs[:8], scs[:8]):
+            probe_absmax = max(probe_absmax, float(np.abs(d.read_ports(0, sc.total_steps)).max()))
+    for d in devs:
+        d.close()
+    # one at a time
+    alone_work, alone_wall = 0, 0.0
+    for sc in scs[::sample_every]:
+        d = tlm.open_scene(sc)
+        d.sync()
+        t1 = time.perf_counter()
+        n = 0
+        while n < sc.total_steps:
+            m = min(256, sc.total_steps - n)
+            d.run(n, m)
+            n += m
+        d.sync()
+        alone_wall += time.perf_counter() - t1
+        alone_work += sc.ncell * sc.total_steps
+        d.close()
+    out = {
+        "grids": count,
+        "gcell_s": round(work / wall / 1e9, 3),
+        "wall_s": round(wall, 3),
+        "cell_updates": work,
+        "tuning": tuning,
+        "one_at_a_time_gcell_s": round(alone_work / alone_wall / 1e9, 3),
+        "one_at_a_time_sample": f"every {sample_every}th grid ({len(scs[::sample_every])} grids)",
+        "port_absmax": probe_absmax,
+    }
+    out["speedup_vs_one_at_a_time"] = round(out["gcell_s"] / out["one_at_a_time_gcell_s"], 3)
+    log(f"[bench] batched_{count}: {out['gcell_s']:.2f} Gcell/s aggregate in {wall:.1f}s "
+        f"(tuning {tuning}); one at a time {out['one_at_a_time_gcell_s']:.2f} Gcell/s")
+    return out
+
+
+def served_run(count=200, rate=50.0, baseline_sample=20, seed=5):
+    """The served mixed-size stream (Poisson arrivals at `rate`/s, axes
+    log-uniform in [32,384], steps U[500,4000], FP16 storage for the requests
+    that tolerate 1e-3, FP32 otherwise) through the resident serving loop.
+    Baseline: `baseline_sample` of the same requests solved one at a time as
+    plain FP32 contexts."""
+    from emsolver_amd import serve, tlm
+    reqs = serve.served_stream(count=count, rate=rate, seed=seed)
+    srv = serve.TlmServer(block_ticks=32, autotune_ticks=2, keep_results=False)
+    done = srv.serve(reqs)
+    summ = serve.summarize(done)
+    summ.update({"rate_per_s": rate, "mean_members": round(srv.stats["mean_members"], 2),
+                 "tuning": srv.stats["tuning"]})
+    alone_work, alone_wall = 0, 0.0
+    for r in reqs[:baseline_sample]:
+        sc = r.build()
+        sc.pulse = "f32"
+        d = tlm.open_scene(sc)
+        d.sync()
+        t1 = time.perf_counter()
+        n = 0
+        while n < sc.total_steps:
+            m = min(256, sc.total_steps - n)
+            d.run(n, m)
+            n += m
+        d.sync()
+        alone_wall += time.perf_counter() - t1
+        alone_work += sc.ncell * sc.total_steps
+        d.close()
+    summ["one_at_a_time_fp32_gcell_s"] = round(alone_work / alone_wall / 1e9, 3)
+    summ["one_at_a_time_sample"] = f"first {baseline_sample} requests, FP32, alone"
+    summ["speedup_vs_one_at_a_time"] = round(summ["gcell_s"] / summ["one_at_a_time_fp32_gcell_s"], 3)
+    for k in ("p50_latency_s", "p99_latency_s", "mean_latency_s", "gcell_s", "makespan_s"):
+        summ[k] = round(summ[k], 4)
+    log(f"[bench] served_{count}: p50 {summ['p50_latency_s']:.2f}s p99 {summ['p99_latency_s']:.2f}s, "
+        f"{summ['gcell_s']:.2f} Gcell/s aggregate (mean {summ['mean_members']} members); "
+        f"one at a time {summ['one_at_a_time_fp32_gcell_s']:.2f} Gcell/s")
+    return summ
+
+
 def cpu_baseline(scene, budget_s=15.0):
     from tests import helpers
     o = helpers.oracle_open(scene.with_steps(10**6))
@@ -147,6 +250,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--extra-steps", type=int, default=200,
                     help="timed steps of each extra 512^3 line")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched and served lines")
+    ap.add_argument("--serve-count", type=int, default=200,
+                    help="requests of the served line (the full stream is 2000)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -209,6 +315,10 @@ def main():
         }
         log(f"[bench] free_space_512: {r['gcells']:.2f} Gcell/s, kernel {r['kernel_ms_avg']:.3f} ms, "
             f"{r['achieved_gbs']:.0f} GB/s = {r['achieved_gbs'] / HBM_PEAK_GBS:.3f} of peak")
+
+    if not args.no_batch:
+        extra["batched_256"] = batched_run(256)
+        extra[f"served_{args.serve_count}"] = served_run(args.serve_count)
 
     cpu = None
     if not args.no_cpu:

@@ -30,7 +30,7 @@ class GridDesc(C.Structure):
 
 class BatchTuning(C.Structure):
     _fields_ = [("planes", C.c_int32), ("block_waves", C.c_int32), ("grid_blocks", C.c_int32),
-                ("graph_ticks", C.c_int32)]
+                ("graph_ticks", C.c_int32), ("planes16", C.c_int32)]
 
 
 class Tuning(C.Structure):

@@ -44,7 +44,7 @@ struct tlm_batch
     std::vector<Member> mem;
     long long tick = 0; // ticks queued so far (the device counter after they ran)
     long long *d_tick = nullptr;
-    tlm_batch_tuning tune = {1, 4, 0, 32};
+    tlm_batch_tuning tune = {1, 4, 0, 32, 8};
     bool dirty = true;
     // device tables, rebuilt when the membership or the tuning changes
     tlm::BatchMember *d_mem = nullptr;
@@ -136,7 +136,8 @@ int rebuild(tlm_batch *b)
         tlm_ctx *c = M.c;
         tlm::BatchMember &d = hm[i];
         std::memset(&d, 0, sizeof(d));
-        const int zc = b->tune.planes > 0 ? std::min(b->tune.planes, c->nz) : c->nz;
+        const int pl = c->pf == TLM_PULSE_F32 ? b->tune.planes : b->tune.planes16;
+        const int zc = pl > 0 ? std::min(pl, c->nz) : c->nz;
         const int nzc = (c->nz + zc - 1) / zc;
         const int strips = (int)(c->Pp / tlm::kWave);
         const int tasks = strips * nzc;
@@ -524,7 +525,7 @@ int tlm_batch_members(tlm_batch *b, tlm_ctx **members, int32_t *remaining, int c
 int tlm_batch_set_tuning(tlm_batch *b, const tlm_batch_tuning *t)
 {
     if (!b || !t || t->planes < 0 || t->block_waves < 0 || t->block_waves > 4 ||
-        t->grid_blocks < 0 || t->graph_ticks < 0)
+        t->grid_blocks < 0 || t->graph_ticks < 0 || t->planes16 < 0)
         return bfail(b, TLM_ERR_ARG, "tlm_batch_set_tuning: bad tuning");
     b->tune = *t;
     b->dirty = true;
@@ -581,38 +582,45 @@ int tlm_batch_autotune(tlm_batch *b, int ticks_per_candidate, tlm_batch_tuning *
     if (b->mem.empty())
         return bfail(b, TLM_ERR_STATE, "tlm_batch_autotune: empty batch");
     const tlm_batch_tuning keep = b->tune;
-    static const int planes[] = {1, 2, 4, 8};
-    static const int waves[] = {2, 4};
-    static const int grids[] = {0, 2048};
+    bool have16 = false, have32 = false;
+    for (const Member &m : b->mem)
+        (m.c->pf == TLM_PULSE_F32 ? have32 : have16) = true;
+    std::vector<tlm_batch_tuning> cands;
+    for (int p : {1, 2, 4, 8})
+        for (int w : {2, 4})
+            for (int g : {0, 2048})
+                for (int p16 : {4, 8, 16})
+                {
+                    if ((!have32 && p != 1) || (!have16 && p16 != 8))
+                        continue;
+                    cands.push_back({p, w, g, 0, p16});
+                }
     tlm_batch_tuning top = keep;
     double top_ns = 1e300;
-    for (int p : planes)
-        for (int w : waves)
-            for (int g : grids)
-            {
-                tlm_batch_tuning t = {p, w, g, 0};
-                b->tune = t;
-                b->dirty = true;
-                if (active_cells(b) == 0)
-                    break;
-                float ms = 0.f;
-                int64_t cu = 0;
-                int rc = tlm_batch_time(b, 1, &ms, &cu); // warm-up (table rebuild)
-                if (!rc)
-                    rc = tlm_batch_time(b, ticks_per_candidate, &ms, &cu);
-                if (rc)
-                {
-                    b->tune = keep;
-                    b->dirty = true;
-                    return rc;
-                }
-                const double ns = cu > 0 ? ms * 1e6 / (double)cu : 1e300;
-                if (ns < top_ns)
-                {
-                    top_ns = ns;
-                    top = t;
-                }
-            }
+    for (const tlm_batch_tuning &t : cands)
+    {
+        if (active_cells(b) == 0)
+            break;
+        b->tune = t;
+        b->dirty = true;
+        float ms = 0.f;
+        int64_t cu = 0;
+        int rc = tlm_batch_time(b, 1, &ms, &cu); // warm-up (table rebuild)
+        if (!rc)
+            rc = tlm_batch_time(b, ticks_per_candidate, &ms, &cu);
+        if (rc)
+        {
+            b->tune = keep;
+            b->dirty = true;
+            return rc;
+        }
+        const double ns = cu > 0 ? ms * 1e6 / (double)cu : 1e300;
+        if (ns < top_ns)
+        {
+            top_ns = ns;
+            top = t;
+        }
+    }
     top.graph_ticks = keep.graph_ticks;
     b->tune = top;
     b->dirty = true;

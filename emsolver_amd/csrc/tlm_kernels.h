@@ -688,12 +688,10 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
                     }
                 }
             }
-            float wm = mc;
-#pragma unroll
-            for (int off = kWave / 2; off > 0; off >>= 1)
-                wm = fmaxf(wm, __shfl_xor(wm, off, kWave));
+            // non-negative floats order like their bit patterns: a DPP max
+            const unsigned wm = __reduce_max_sync(~0ull, __float_as_uint(mc));
             if (lane == 0)
-                a.mpart[(size_t)k * a.strips + t] = wm;
+                a.mpart[(size_t)k * a.strips + t] = __uint_as_float(wm);
             zpo0 = o[10];
             zpo1 = o[11];
             pecPrev = pec;

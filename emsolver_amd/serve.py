@@ -1,0 +1,230 @@
+"""Resident serving loop: TLM simulations of different sizes and step counts
+arrive over time, are admitted against free HBM into one running batch
+(continuous batching: members join and leave between blocks of ticks while
+the others keep running), and complete with their probe and port records.
+
+The device work is the batched engine of the C ABI (``tlm_batch_*``): a block
+of ticks is one captured HIP graph. A member's result is the same bits alone
+and in the mix (each grid's arithmetic does not depend on its neighbours in
+the batch); tests/test_gpu_serve.py checks exactly that.
+"""
+from __future__ import annotations
+
+import collections
+import dataclasses
+import math
+import threading
+import time
+
+import numpy as np
+
+from . import scenes, tlm
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    scene: object          # scenes.Scene (built lazily by `build` when None)
+    arrival: float         # seconds after the start of the stream
+    tol: float = 1e-5      # tolerance the client asks for; 1e-3 allows FP16 storage
+    build: object = None   # callable returning the Scene (generated off the serving loop)
+
+
+@dataclasses.dataclass
+class Completion:
+    rid: int
+    arrival: float
+    admitted: float
+    finished: float
+    cells: int
+    steps: int
+    pulse: str
+    probes: np.ndarray | None = None
+    ports: np.ndarray | None = None
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+def storage_for(tol: float) -> str:
+    """FP16 pulse storage where the client tolerates 1e-3, else FP32."""
+    return "f16" if tol >= 1e-3 else "f32"
+
+
+class TlmServer:
+    """One device, one batch, requests admitted by free HBM."""
+
+    def __init__(self, device=0, block_ticks=32, hbm_reserve=4 << 30, max_members=1024,
+                 tuning=None, autotune_ticks=0, keep_results=True):
+        self.device = device
+        self.block_ticks = int(block_ticks)
+        self.hbm_reserve = int(hbm_reserve)
+        self.max_members = int(max_members)
+        self.tuning = tuning
+        self.autotune_ticks = int(autotune_ticks)
+        self.keep_results = keep_results
+        self.stats = {}
+
+    def _need_bytes(self, sc) -> int:
+        stubs = bool((sc.mat >= scenes.MAT_DIEL0).any())
+        b = tlm.estimate_bytes(sc.nx, sc.ny, sc.nz, sc.pulse, stubs)
+        if sc.pulse != "f32" and sc.fp32_halo >= 0:
+            # FP32 slabs near the sources take 8 bytes per pair instead of 4
+            b += int(2 * 12 * sc.nx * sc.ny * min(sc.nz, 2 * sc.fp32_halo + 2) * 2)
+        steps = sc.total_steps
+        b += steps * (len(sc.probes) * 12 + len(sc.ports) * 8) + (1 << 20)
+        return b
+
+    def serve(self, requests, realtime=True, prefetch=8):
+        """Serve `requests` (sorted by arrival). With realtime=False arrivals
+        are ignored (every request is available at once, admission still by
+        memory and max_members). Returns the completions in finishing order."""
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        # scenes are generated ahead of the loop on a worker thread
+        ready = {}
+        cond = threading.Condition()
+
+        def producer():
+            for r in reqs:
+                sc = r.scene if r.scene is not None else r.build()
+                with cond:
+                    while len(ready) >= prefetch:
+                        cond.wait()
+                    ready[r.rid] = sc
+                    cond.notify_all()
+
+        th = threading.Thread(target=producer, daemon=True)
+        th.start()
+
+        def take(rid):
+            with cond:
+                while rid not in ready:
+                    cond.wait()
+                sc = ready.pop(rid)
+                cond.notify_all()
+                return sc
+
+        batch = tlm.TlmBatch(self.device)
+        if self.tuning:
+            batch.set_tuning(**self.tuning)
+        t0 = time.perf_counter()
+        now = (lambda: time.perf_counter() - t0) if realtime else (lambda: 0.0)
+        arrivals = collections.deque(reqs)
+        queue = collections.deque()
+        live = {}       # TlmDevice -> (request, scene, admitted time)
+        done = []
+        tuned = False
+        ticks = 0
+        member_ticks = 0
+        while arrivals or queue or live:
+            t = now()
+            while arrivals and arrivals[0].arrival <= t:
+                queue.append(arrivals.popleft())
+            # admission: FIFO, against free device memory
+            while queue and len(live) < self.max_members:
+                r = queue[0]
+                sc = take(r.rid)
+                need = self._need_bytes(sc)
+                free, _ = tlm.device_memory(self.device)
+                if live and need + self.hbm_reserve > free:
+                    with cond:  # put it back; try again after members leave
+                        ready[r.rid] = sc
+                        cond.notify_all()
+                    break
+                queue.popleft()
+                dev = tlm.open_scene(sc, self.device)
+                batch.add(dev, sc.total_steps)
+                live[dev] = (r, sc, now())
+            if not live:
+                if arrivals and realtime:
+                    time.sleep(max(0.0, min(0.05, arrivals[0].arrival - now())))
+                continue
+            if self.autotune_ticks and not tuned and len(live) >= 2:
+                batch.autotune(self.autotune_ticks)
+                tuned = True
+            batch.run(self.block_ticks)
+            ticks += self.block_ticks
+            member_ticks += self.block_ticks * len(live)
+            batch.sync()
+            for dev, left in batch.remaining().items():
+                if left > 0:
+                    continue
+                r, sc, adm = live.pop(dev)
+                batch.remove(dev)
+                c = Completion(r.rid, r.arrival, adm, now(), sc.ncell, sc.total_steps, sc.pulse)
+                if self.keep_results:
+                    if dev.nprobe:
+                        c.probes = dev.read_probes(0, sc.total_steps)
+                    if dev.nport:
+                        c.ports = dev.read_ports(0, sc.total_steps)
+                dev.close()
+                if not realtime:
+                    c.finished = time.perf_counter() - t0
+                done.append(c)
+        self.stats = {"ticks": ticks, "mean_members": member_ticks / max(1, ticks),
+                      "wall_s": time.perf_counter() - t0, "tuning": batch.get_tuning()}
+        batch.close()
+        return done
+
+
+def summarize(done, wall_s=None) -> dict:
+    """p50 / p99 latency and aggregate Gcell-updates/s of a served stream."""
+    lat = np.array([c.latency for c in done])
+    work = float(sum(c.cells * c.steps for c in done))
+    t_first = min(c.arrival for c in done)
+    t_last = max(c.finished for c in done)
+    span = wall_s if wall_s is not None else (t_last - t_first)
+    return {
+        "requests": len(done),
+        "p50_latency_s": float(np.percentile(lat, 50)),
+        "p99_latency_s": float(np.percentile(lat, 99)),
+        "mean_latency_s": float(lat.mean()),
+        "gcell_s": work / span / 1e9,
+        "cell_updates": work,
+        "makespan_s": span,
+        "f16_requests": int(sum(c.pulse == "f16" for c in done)),
+    }
+
+
+def served_stream(count=2000, rate=50.0, lo=32, hi=384, steps=(500, 4000), seed=5,
+                  f16_share=0.5, nprobes=2):
+    """The served workload: Poisson arrivals at `rate`/s, each axis log-uniform
+    in [lo, hi], the dielectric-box background with a centre-fed dipole of
+    seeded length, steps uniform in [steps], per-request tolerance 1e-3
+    (FP16 storage) or 1e-5 (FP32) with a seeded split. Scenes are built on
+    demand (Request.build)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for rid in range(count):
+        t += float(rng.exponential(1.0 / rate))
+        dims = tuple(int(round(math.exp(rng.uniform(math.log(lo), math.log(hi))))) for _ in range(3))
+        frac = float(rng.uniform(0.3, 0.7))
+        st = int(rng.integers(steps[0], steps[1] + 1))
+        tol = 1e-3 if rng.random() < f16_share else 1e-5
+        sseed = int(rng.integers(1 << 31))
+
+        def build(dims=dims, frac=frac, st=st, tol=tol, sseed=sseed, rid=rid):
+            return request_scene(dims, frac, st, storage_for(tol), sseed, nprobes, f"req{rid}")
+
+        out.append(Request(rid, None, t, tol, build))
+    return out
+
+
+def request_scene(dims, length_frac, steps, pulse, seed, nprobes=2, name="req"):
+    nx, ny, nz = dims
+    sc = scenes.dipole_scene(nx, ny, nz, length_frac=length_frac, steps=steps,
+                             background_boxes=8, seed=seed)
+    sc.faces = False
+    sc.pulse = pulse
+    rng = np.random.default_rng(seed + 1)
+    probes = []
+    while len(probes) < nprobes:
+        c = sc.cell(int(rng.integers(1, nx - 1)), int(rng.integers(1, ny - 1)),
+                    int(rng.integers(1, nz - 1)))
+        if sc.mat[c] != scenes.MAT_PEC and c not in probes:
+            probes.append(c)
+    sc.probes = np.array(probes, np.int64)
+    sc.name = name
+    return sc
