@@ -1,12 +1,18 @@
-// tlm_kernels.h - device kernels of the TLM time step (FP32 pulses).
+// tlm_kernels.h - device kernels of the TLM time step.
 //
-// Layout in HBM (set up in GpuTlmHip.hip):
-//   * pulses: 12 planes of P floats each (structure of arrays). Inside a
-//     plane, z-slab k starts at k*Pp with Pp = roundup(nx*ny, 64), so every
-//     wavefront's 64 consecutive cells are one aligned 256-byte run; the
-//     cell (i, j, k) lives at k*Pp + j*nx + i. Port p of that cell is at
-//     p*P + k*Pp + j*nx + i. The slab padding is never read as a neighbour.
-//   * two such buffers (A = incident pulses of step n, B = of step n+1).
+// Layout in HBM (set up in GpuTlmHip.hip). Cells are addressed in a
+// slab-padded space: z-slab k starts at k*Pp with Pp = roundup(nx*ny, 64),
+// so the cell (i, j, k) is k*Pp + j*nx + i and every wavefront's 64
+// consecutive cells share one aligned block. The padding is never a
+// neighbour.
+//   * FP32 pulses: a slab is Pp/64 blocks of 3 KB, block b holding the 12
+//     ports of its 64 cells port after port (port p of lane l at byte
+//     p*256 + l*4 of the block), so the 24 loads and stores of a wave touch
+//     two contiguous 3 KB runs.
+//   * 16-bit pulses: 6 pair planes (ports 2q, 2q+1 of a cell side by side),
+//     each slab stored as 16-bit codes (4 bytes per cell and pair) or, near
+//     soft sources, as FP32 (8 bytes).
+//   * two such pulse buffers (A = incident pulses of step n, B = of n+1).
 //   * stubs: 3 planes of P floats, updated in place, dielectric cells only.
 //   * cell code: 16 bits per cell address: material id in bits 0-7 (0 air,
 //     1 PEC, >= 2 dielectric), then one bit per face telling whether the
@@ -113,54 +119,6 @@ enum : unsigned
     NB_ZP = 1u << 13
 };
 
-// One slab as seen by one lane: byte offset of the lane's cell inside a
-// plane, element size, format and scales (wave-uniform apart from bo).
-struct SlabRef
-{
-    unsigned bo, es;
-    bool h16;
-    float ds, isc;
-};
-template <int PF> __device__ __forceinline__ SlabRef slab_ref(const StepArgs &a, int k, unsigned mo, int f)
-{
-    if (PF == PF_F32)
-        return {mo * 4u, 4u, false, 1.0f, 1.0f};
-    const uint2 s = a.slab[k];
-    // the slab's format is the same for the whole wave: a scalar branch
-    const bool h = __builtin_amdgcn_readfirstlane(s.y) != 0u;
-    const unsigned es = h ? 2u : 4u;
-    return {s.x + (unsigned)f * es, es, h, a.dsc[k], a.esc[k]};
-}
-template <int PF> __device__ __forceinline__ float ldp(const char *base, unsigned boff, const SlabRef &s)
-{
-    if (PF != PF_F32 && s.h16)
-        return dec16<PF>(*reinterpret_cast<const unsigned short *>(base + boff)) * s.ds;
-    return ldo(base, boff);
-}
-// Same with the format known at compile time (the slab's own accesses).
-template <int PF, bool H16> __device__ __forceinline__ float ldh(const char *base, unsigned boff, const SlabRef &s)
-{
-    if (H16)
-        return dec16<PF>(*reinterpret_cast<const unsigned short *>(base + boff)) * s.ds;
-    return ldo(base, boff);
-}
-template <int PF, bool NT, bool H16>
-__device__ __forceinline__ void sth(char *base, unsigned boff, const SlabRef &s, float v)
-{
-    if (H16)
-        *reinterpret_cast<unsigned short *>(base + boff) = enc16<PF>(v * s.isc);
-    else
-        sto<NT>(base, boff, v);
-}
-template <int PF, bool NT>
-__device__ __forceinline__ void stp(char *base, unsigned boff, const SlabRef &s, float v)
-{
-    if (PF != PF_F32 && s.h16)
-        *reinterpret_cast<unsigned short *>(base + boff) = enc16<PF>(v * s.isc);
-    else
-        sto<NT>(base, boff, v);
-}
-
 // Workgroups are dispatched to the 8 XCDs round robin. Each XCD gets one
 // contiguous range of task groups, walked by its own workgroups in a
 // grid-stride loop, so neighbouring strips share an L2 (which keeps the
@@ -183,6 +141,16 @@ __device__ __forceinline__ XcdRange xcd_range(int b, int G, int ngroups)
     return {start + local, start + count, gper};
 }
 
+// FP32 pulse layout: each z-slab is a run of 64-cell blocks, a block holding
+// the 12 ports of its 64 cells one after another (port p of block lane l at
+// p*256 + l*4 bytes, 3 KB per block). A wavefront's 12 loads and 12 stores
+// for its strip touch one contiguous 3 KB block, and a slab is Pp*48 bytes.
+// Byte offset of in-slab cell g inside its slab:
+__device__ __forceinline__ unsigned blk_off(unsigned g)
+{
+    return (g >> 6) * 3072u + (g & 63u) * 4u;
+}
+
 // Fused scatter + connect. A wavefront owns a strip of 64 consecutive cells
 // of one z-slab (lane = cell) and marches over zc slabs.
 //
@@ -198,29 +166,22 @@ __device__ __forceinline__ XcdRange xcd_range(int b, int G, int ngroups)
 // The owner writes its own slot instead whenever the link is broken: own
 // cell PEC (value kept, times rho on an outer face), outer face (rho * out),
 // neighbour PEC (-out). This is connect(), TlmSolver.cpp:563-583.
-//
-// 16-bit formats: a value is rounded once, where it is stored, with the
-// scale of the slab it is stored in; each wave also records the largest
-// |value| its cells send (|out| and new stubs of non-PEC cells, |in| of PEC
-// cells), which bounds the scales of the next steps.
 // One wave task: a 64-cell strip of slabs [k0, k1).
 template <bool STUBS, bool ENERGY, bool NT, int PF>
 __device__ __forceinline__ void strip_task(const StepArgs &a, int task, int lane, double &esum)
 {
+    static_assert(PF == PF_F32, "the 16-bit formats use strip_task16");
     const int tz = task / a.strips, t = task - tz * a.strips;
     const int f = t * kWave + lane; // cell inside the slab
     const bool act = f < a.nxy;
     const int j = f / a.nx, i = f - j * a.nx;
     const int k0 = tz * a.zc, k1 = min(a.nz, k0 + a.zc);
     const float rho = a.rho;
-    const char *A[12];
-    char *B[12];
-#pragma unroll
-    for (int p = 0; p < 12; ++p)
-    {
-        A[p] = a.A + p * a.PB;
-        B[p] = a.B + p * a.PB;
-    }
+    const size_t slabB = (size_t)a.Pp * 48u;
+    // in-slab byte offsets of this cell and of its x / y neighbours
+    const unsigned lo = (unsigned)t * 3072u + (unsigned)lane * 4u;
+    const unsigned loL = blk_off((unsigned)f - 1u), loR = blk_off((unsigned)f + 1u);
+    const unsigned loYm = blk_off((unsigned)(f - a.nx)), loYp = blk_off((unsigned)(f + a.nx));
     // x neighbour inside the wave; otherwise (strip seam) the two cells push
     const bool hasL = act && lane > 0 && i > 0;
     const bool hasR = act && lane < kWave - 1 && i < a.nx - 1;
@@ -228,214 +189,173 @@ __device__ __forceinline__ void strip_task(const StepArgs &a, int task, int lane
 
     float zpo0 = 0.f, zpo1 = 0.f; // previous slab: reflected z+ pulses
     bool pecPrev = false;
-    unsigned mo = (unsigned)(k0 * a.Pp + f); // cell index in the slab-padded space
-    SlabRef sp = {0u, 4u, false, 1.0f, 1.0f}; // previous slab
+    unsigned mo = (unsigned)(k0 * a.Pp + f); // cell index in the slab-padded space (codes, stubs)
+    const char *Ak = a.A + (size_t)k0 * slabB;
+    char *Bk = a.B + (size_t)k0 * slabB;
 
-    for (int k = k0; k < k1; ++k, mo += (unsigned)a.Pp)
+    for (int k = k0; k < k1; ++k, mo += (unsigned)a.Pp, Ak += slabB, Bk += slabB)
     {
-        const SlabRef s = slab_ref<PF>(a, k, mo, f);
-        // the slab's own loads and stores use one storage format, chosen by a
-        // wave-uniform branch around the whole slab
-        auto slab_body = [&](auto h16_tag) {
-            constexpr bool H16 = decltype(h16_tag)::value;
-                const unsigned bo = s.bo;
-                const unsigned by = (unsigned)a.nx * s.es;
-                float o[12];
-                float mc = 0.f; // 16-bit formats: largest |value| this cell sends
-                bool pec = true;
-                unsigned code = 0;
-                if (act)
+        float o[12];
+        bool pec = true;
+        unsigned code = 0;
+        if (act)
+        {
+            code = ldc(a.code, mo);
+            const unsigned m = code & 0xFFu;
+            pec = (m == 1u);
+            float v[12];
+#pragma unroll
+            for (int p = 0; p < 12; ++p)
+                v[p] = ldo(Ak + p * 256, lo);
+            if (ENERGY)
+            {
+#pragma unroll
+                for (int p = 0; p < 12; ++p)
+                    esum += (double)v[p] * (double)v[p];
+            }
+            if (STUBS && m >= 2u)
+            {
+                float *S0 = a.Vs, *S1 = a.Vs + a.P, *S2 = a.Vs + 2 * a.P;
+                const unsigned bs = mo * 4u;
+                float st[3] = {ldo((const char *)S0, bs), ldo((const char *)S1, bs),
+                               ldo((const char *)S2, bs)};
+                const float4 pr = a.props[m - 2u];
+                scn::scatter_diel(v, st, pr.x, pr.z, o);
+                sto<false>((char *)S0, bs, st[0]);
+                sto<false>((char *)S1, bs, st[1]);
+                sto<false>((char *)S2, bs, st[2]);
+            }
+            else
+                scn::scatter_air(v, o);
+            if (pec)
+            {
+                // a PEC cell keeps its incident pulses: o carries them
+#pragma unroll
+                for (int p = 0; p < 12; ++p)
+                    o[p] = v[p];
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int p = 0; p < 12; ++p)
+                o[p] = 0.f;
+        }
+        // every lane takes part in the shuffles
+        const float L0 = __shfl_up(o[2], 1, kWave), L1 = __shfl_up(o[3], 1, kWave);
+        const float R0 = __shfl_down(o[0], 1, kWave), R1 = __shfl_down(o[1], 1, kWave);
+        if (act)
+        {
+            // ---- x faces: ports 0,1 (x-) and 2,3 (x+) ----
+            {
+                const bool pecL = (code & NB_XM) != 0u, pecR = (code & NB_XP) != 0u;
+#pragma unroll
+                for (int pol = 0; pol < 2; ++pol)
                 {
-                    code = ldc(a.code, mo);
-                    const unsigned m = code & 0xFFu;
-                    pec = (m == 1u);
-                    float v[12];
-        #pragma unroll
-                    for (int p = 0; p < 12; ++p)
-                        v[p] = ldh<PF, H16>(A[p], bo, s);
-                    if (ENERGY)
+                    const float oo = o[0 + pol];
+                    if (pec)
+                        sto<NT>(Bk + (0 + pol) * 256, lo, bx0 ? rho * oo : oo);
+                    else if (bx0)
+                        sto<NT>(Bk + (0 + pol) * 256, lo, rho * oo);
+                    else if (pecL)
+                        sto<NT>(Bk + (0 + pol) * 256, lo, -oo);
+                    else if (hasL)
+                        sto<NT>(Bk + (0 + pol) * 256, lo, pol ? L1 : L0);
+                    else // seam with a live neighbour: it fills my slot, I fill its
+                        sto<NT>(Bk + (2 + pol) * 256, loL, oo);
+
+                    const float op = o[2 + pol];
+                    if (pec)
+                        sto<NT>(Bk + (2 + pol) * 256, lo, bx1 ? rho * op : op);
+                    else if (bx1)
+                        sto<NT>(Bk + (2 + pol) * 256, lo, rho * op);
+                    else if (pecR)
+                        sto<NT>(Bk + (2 + pol) * 256, lo, -op);
+                    else if (hasR)
+                        sto<NT>(Bk + (2 + pol) * 256, lo, pol ? R1 : R0);
+                    else
+                        sto<NT>(Bk + (0 + pol) * 256, loR, op);
+                }
+            }
+            // ---- y faces: ports 4,5 (y-) and 6,7 (y+), sender push ----
+            {
+                const bool pecLo = (code & NB_YM) != 0u, pecHi = (code & NB_YP) != 0u;
+#pragma unroll
+                for (int pol = 0; pol < 2; ++pol)
+                {
+                    const float ol = o[4 + pol], oh = o[6 + pol];
+                    if (pec)
                     {
-        #pragma unroll
-                        for (int p = 0; p < 12; ++p)
-                            esum += (double)v[p] * (double)v[p];
-                    }
-                    if (STUBS && m >= 2u)
-                    {
-                        float *S0 = a.Vs, *S1 = a.Vs + a.P, *S2 = a.Vs + 2 * a.P;
-                        const unsigned bs = mo * 4u;
-                        float st[3] = {ldo((const char *)S0, bs), ldo((const char *)S1, bs),
-                                       ldo((const char *)S2, bs)};
-                        const float4 pr = a.props[m - 2u];
-                        scn::scatter_diel(v, st, pr.x, pr.z, o);
-                        sto<false>((char *)S0, bs, st[0]);
-                        sto<false>((char *)S1, bs, st[1]);
-                        sto<false>((char *)S2, bs, st[2]);
-                        if (PF != PF_F32)
-                            mc = fmaxf(fmaxf(fabsf(st[0]), fabsf(st[1])), fabsf(st[2]));
+                        sto<NT>(Bk + (4 + pol) * 256, lo, by0 ? rho * ol : ol);
+                        sto<NT>(Bk + (6 + pol) * 256, lo, by1 ? rho * oh : oh);
                     }
                     else
-                        scn::scatter_air(v, o);
-                    if (PF != PF_F32)
                     {
-        #pragma unroll
-                        for (int p = 0; p < 12; ++p)
-                            mc = fmaxf(mc, fabsf(pec ? v[p] : o[p]));
+                        if (by0)
+                            sto<NT>(Bk + (4 + pol) * 256, lo, rho * ol);
+                        else if (pecLo)
+                            sto<NT>(Bk + (4 + pol) * 256, lo, -ol);
+                        else
+                            sto<NT>(Bk + (6 + pol) * 256, loYm, ol);
+                        if (by1)
+                            sto<NT>(Bk + (6 + pol) * 256, lo, rho * oh);
+                        else if (pecHi)
+                            sto<NT>(Bk + (6 + pol) * 256, lo, -oh);
+                        else
+                            sto<NT>(Bk + (4 + pol) * 256, loYp, oh);
                     }
                 }
+            }
+            // ---- z faces: ports 8,9 (z-) and 10,11 (z+) ----
+#pragma unroll
+            for (int pol = 0; pol < 2; ++pol)
+            {
+                const float ol = o[8 + pol];
+                if (k > k0)
+                {
+                    // pull: slab k-1 is in registers
+                    const float zp = pol ? zpo1 : zpo0;
+                    if (pec)
+                        sto<NT>(Bk + (8 + pol) * 256, lo, ol);
+                    else
+                        sto<NT>(Bk + (8 + pol) * 256, lo, pecPrev ? -ol : zp);
+                    // slab k-1's z+ slot: its kept pulse if it is PEC (zp holds
+                    // it), else what this cell sends down
+                    sto<NT>(Bk - slabB + (10 + pol) * 256, lo, pecPrev ? zp : (pec ? -zp : ol));
+                }
+                else if (k == 0)
+                    sto<NT>(Bk + (8 + pol) * 256, lo, rho * ol);
                 else
                 {
-        #pragma unroll
-                    for (int p = 0; p < 12; ++p)
-                        o[p] = 0.f;
+                    const bool pecZ = (code & NB_ZM) != 0u;
+                    if (pec)
+                        sto<NT>(Bk + (8 + pol) * 256, lo, ol);
+                    else if (pecZ)
+                        sto<NT>(Bk + (8 + pol) * 256, lo, -ol);
+                    else
+                        sto<NT>(Bk - slabB + (10 + pol) * 256, lo, ol);
                 }
-                // every lane takes part in the shuffles
-                const float L0 = __shfl_up(o[2], 1, kWave), L1 = __shfl_up(o[3], 1, kWave);
-                const float R0 = __shfl_down(o[0], 1, kWave), R1 = __shfl_down(o[1], 1, kWave);
-                if (act)
+                if (k == k1 - 1)
                 {
-                    // ---- x faces: ports 0,1 (x-) and 2,3 (x+) ----
+                    const float oh = o[10 + pol];
+                    if (k == a.nz - 1)
+                        sto<NT>(Bk + (10 + pol) * 256, lo, rho * oh);
+                    else
                     {
-                        const bool pecL = (code & NB_XM) != 0u, pecR = (code & NB_XP) != 0u;
-        #pragma unroll
-                        for (int pol = 0; pol < 2; ++pol)
-                        {
-                            const float oo = o[0 + pol];
-                            if (pec)
-                            {
-                                const float old = ldh<PF, H16>(A[0 + pol], bo, s);
-                                sth<PF, NT, H16>(B[0 + pol], bo, s, bx0 ? rho * old : old);
-                            }
-                            else if (bx0)
-                                sth<PF, NT, H16>(B[0 + pol], bo, s, rho * oo);
-                            else if (pecL)
-                                sth<PF, NT, H16>(B[0 + pol], bo, s, -oo);
-                            else if (hasL)
-                                sth<PF, NT, H16>(B[0 + pol], bo, s, pol ? L1 : L0);
-                            else // seam with a live neighbour: it fills my slot, I fill its
-                                sth<PF, NT, H16>(B[2 + pol], bo - s.es, s, oo);
-
-                            const float op = o[2 + pol];
-                            if (pec)
-                            {
-                                const float old = ldh<PF, H16>(A[2 + pol], bo, s);
-                                sth<PF, NT, H16>(B[2 + pol], bo, s, bx1 ? rho * old : old);
-                            }
-                            else if (bx1)
-                                sth<PF, NT, H16>(B[2 + pol], bo, s, rho * op);
-                            else if (pecR)
-                                sth<PF, NT, H16>(B[2 + pol], bo, s, -op);
-                            else if (hasR)
-                                sth<PF, NT, H16>(B[2 + pol], bo, s, pol ? R1 : R0);
-                            else
-                                sth<PF, NT, H16>(B[0 + pol], bo + s.es, s, op);
-                        }
-                    }
-                    // ---- y faces: ports 4,5 (y-) and 6,7 (y+), sender push ----
-                    {
-                        const bool pecLo = (code & NB_YM) != 0u, pecHi = (code & NB_YP) != 0u;
-        #pragma unroll
-                        for (int pol = 0; pol < 2; ++pol)
-                        {
-                            const float ol = o[4 + pol], oh = o[6 + pol];
-                            if (pec)
-                            {
-                                const float vl = ldh<PF, H16>(A[4 + pol], bo, s), vh = ldh<PF, H16>(A[6 + pol], bo, s);
-                                sth<PF, NT, H16>(B[4 + pol], bo, s, by0 ? rho * vl : vl);
-                                sth<PF, NT, H16>(B[6 + pol], bo, s, by1 ? rho * vh : vh);
-                            }
-                            else
-                            {
-                                if (by0)
-                                    sth<PF, NT, H16>(B[4 + pol], bo, s, rho * ol);
-                                else if (pecLo)
-                                    sth<PF, NT, H16>(B[4 + pol], bo, s, -ol);
-                                else
-                                    sth<PF, NT, H16>(B[6 + pol], bo - by, s, ol);
-                                if (by1)
-                                    sth<PF, NT, H16>(B[6 + pol], bo, s, rho * oh);
-                                else if (pecHi)
-                                    sth<PF, NT, H16>(B[6 + pol], bo, s, -oh);
-                                else
-                                    sth<PF, NT, H16>(B[4 + pol], bo + by, s, oh);
-                            }
-                        }
-                    }
-                    // ---- z faces: ports 8,9 (z-) and 10,11 (z+) ----
-        #pragma unroll
-                    for (int pol = 0; pol < 2; ++pol)
-                    {
-                        const float ol = o[8 + pol];
-                        if (k > k0)
-                        {
-                            // pull: slab k-1 is in registers
-                            const float zp = pol ? zpo1 : zpo0;
-                            if (pec)
-                                sth<PF, NT, H16>(B[8 + pol], bo, s, ldh<PF, H16>(A[8 + pol], bo, s));
-                            else
-                                sth<PF, NT, H16>(B[8 + pol], bo, s, pecPrev ? -ol : zp);
-                            // slab k-1's z+ slot
-                            if (pecPrev)
-                                stp<PF, NT>(B[10 + pol], sp.bo, sp, ldp<PF>(A[10 + pol], sp.bo, sp));
-                            else
-                                stp<PF, NT>(B[10 + pol], sp.bo, sp, pec ? -zp : ol);
-                        }
-                        else if (k == 0)
-                        {
-                            sth<PF, NT, H16>(B[8 + pol], bo, s, rho * (pec ? ldh<PF, H16>(A[8 + pol], bo, s) : ol));
-                        }
+                        const bool pecZ = (code & NB_ZP) != 0u;
+                        if (pec)
+                            sto<NT>(Bk + (10 + pol) * 256, lo, oh);
+                        else if (pecZ)
+                            sto<NT>(Bk + (10 + pol) * 256, lo, -oh);
                         else
-                        {
-                            const bool pecLo = (code & NB_ZM) != 0u;
-                            if (pec)
-                                sth<PF, NT, H16>(B[8 + pol], bo, s, ldh<PF, H16>(A[8 + pol], bo, s));
-                            else if (pecLo)
-                                sth<PF, NT, H16>(B[8 + pol], bo, s, -ol);
-                            else
-                            {
-                                const SlabRef sm = slab_ref<PF>(a, k - 1, mo - (unsigned)a.Pp, f);
-                                stp<PF, NT>(B[10 + pol], sm.bo, sm, ol);
-                            }
-                        }
-                        if (k == k1 - 1)
-                        {
-                            const float oh = o[10 + pol];
-                            if (k == a.nz - 1)
-                                sth<PF, NT, H16>(B[10 + pol], bo, s,
-                                            rho * (pec ? ldh<PF, H16>(A[10 + pol], bo, s) : oh));
-                            else
-                            {
-                                const bool pecHi = (code & NB_ZP) != 0u;
-                                if (pec)
-                                    sth<PF, NT, H16>(B[10 + pol], bo, s, ldh<PF, H16>(A[10 + pol], bo, s));
-                                else if (pecHi)
-                                    sth<PF, NT, H16>(B[10 + pol], bo, s, -oh);
-                                else
-                                {
-                                    const SlabRef sq = slab_ref<PF>(a, k + 1, mo + (unsigned)a.Pp, f);
-                                    stp<PF, NT>(B[8 + pol], sq.bo, sq, oh);
-                                }
-                            }
-                        }
+                            sto<NT>(Bk + slabB + (8 + pol) * 256, lo, oh);
                     }
                 }
-                if (PF != PF_F32)
-                {
-                    float wm = mc;
-        #pragma unroll
-                    for (int off = kWave / 2; off > 0; off >>= 1)
-                        wm = fmaxf(wm, __shfl_xor(wm, off, kWave));
-                    if (lane == 0)
-                        a.mpart[(size_t)k * a.strips + t] = wm;
-                }
-                zpo0 = o[10];
-                zpo1 = o[11];
-                pecPrev = pec;
-                sp = s;
-        };
-        if (PF != PF_F32 && s.h16)
-            slab_body(std::true_type{});
-        else
-            slab_body(std::false_type{});
+            }
+        }
+        zpo0 = o[10];
+        zpo1 = o[11];
+        pecPrev = pec;
     }
 }
 
@@ -764,13 +684,17 @@ struct PulseView
     const float *sc;  // per slab 2^E (current step)
     const float *isc; // per slab 2^-E
 };
-// FP32 contexts: 12 planes of floats. 16-bit contexts: 6 pair planes
+// FP32 contexts: 64-cell blocks of 12 ports per slab. 16-bit contexts: 6 pair planes
 // (plane p/2, half p&1), a 16-bit slab holding two codes per cell, an FP32
 // slab a float2.
 __device__ __forceinline__ float pv_get(const PulseView &v, int p, long long c)
 {
     if (v.pf == PF_F32)
-        return reinterpret_cast<const float *>(v.base + p * v.PB)[c];
+    {
+        const long long k = c / v.Pp, f = c - k * v.Pp;
+        return *reinterpret_cast<const float *>(v.base + k * v.Pp * 48 + p * 256 +
+                                                blk_off((unsigned)f));
+    }
     const char *plane = v.base + (p >> 1) * v.PB;
     const long long k = c / v.Pp, f = c - k * v.Pp;
     const uint2 s = v.slab[k];
@@ -785,7 +709,8 @@ __device__ __forceinline__ float pv_set(const PulseView &v, int p, long long c, 
 {
     if (v.pf == PF_F32)
     {
-        reinterpret_cast<float *>(v.base + p * v.PB)[c] = x;
+        const long long k = c / v.Pp, f = c - k * v.Pp;
+        *reinterpret_cast<float *>(v.base + k * v.Pp * 48 + p * 256 + blk_off((unsigned)f)) = x;
         return 0.f;
     }
     char *plane = v.base + (p >> 1) * v.PB;
