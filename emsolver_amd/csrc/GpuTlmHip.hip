@@ -96,6 +96,7 @@ int launch_step(tlm_ctx *c, int n)
     a.B = c->B;
     a.Vs = c->Vs;
     a.code = c->code;
+    a.uair = c->uair;
     a.props = c->props;
     a.epart = c->epart;
     a.slab = c->slab;
@@ -666,6 +667,23 @@ int tlm_upload(tlm_ctx *c, const uint8_t *mat, const float *yhat, const float *g
             }
     }
     HIPCHK(c, hipMemcpy(c->code, code.data(), code.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    {
+        // blocks whose 64 codes are all 0 (the slab padding is coded PEC, so a
+        // block with padding never qualifies)
+        const size_t nblk = (size_t)c->P / tlm::kWave;
+        std::vector<uint8_t> ua(nblk, 0);
+        for (size_t b = 0; b < nblk; ++b)
+        {
+            const uint16_t *q = &code[b * tlm::kWave];
+            uint16_t any = 0;
+            for (int l = 0; l < tlm::kWave; ++l)
+                any |= q[l];
+            ua[b] = any == 0;
+        }
+        int rc0 = upload(c, &c->uair, ua.data(), ua.size());
+        if (rc0)
+            return rc0;
+    }
     // per-material (yh, gh, den), den = 4 + yh + gh as in TlmSolver.cpp:90
     std::vector<float4> props(std::max(1, nmat));
     for (int i = 0; i < nmat; ++i)

@@ -147,6 +147,7 @@ int rebuild(tlm_batch *b)
         a.B = nullptr;
         a.Vs = c->Vs;
         a.code = c->code;
+        a.uair = c->uair;
         a.props = c->props;
         a.epart = nullptr;
         a.slab = c->slab;

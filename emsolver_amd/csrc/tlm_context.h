@@ -32,6 +32,7 @@ struct tlm_ctx
     char *A = nullptr, *B = nullptr;
     float *Vs = nullptr;
     uint16_t *code = nullptr; // cell code: material + neighbour-PEC bits
+    uint8_t *uair = nullptr;  // per 64-cell block: all codes 0 (air, no PEC neighbour)
     float4 *props = nullptr;
     int nmat = 0;
     bool stubs = false, uploaded = false;

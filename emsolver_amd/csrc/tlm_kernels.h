@@ -72,6 +72,7 @@ struct StepArgs
     char *B;              // incident pulses of step n+1
     float *Vs;            // stubs: 3 planes of P floats
     const uint16_t *code; // cell code (material + neighbour-PEC bits)
+    const uint8_t *uair;  // per 64-cell block (slab-major): 1 if all its codes are 0
     const float4 *props;  // per dielectric id: (yh, gh, den, 0)
     double *epart;        // energy partial per wave task
     // 16-bit formats only
@@ -198,15 +199,22 @@ __device__ __forceinline__ void strip_task(const StepArgs &a, int task, int lane
         float o[12];
         bool pec = true;
         unsigned code = 0;
+        // blocks of plain air cells with no PEC neighbour skip the code load
+        const bool uair = a.uair[(size_t)k * a.strips + t] != 0;
         if (act)
         {
-            code = ldc(a.code, mo);
-            const unsigned m = code & 0xFFu;
-            pec = (m == 1u);
             float v[12];
+            // load order measured per variant: code first without stubs,
+            // pulses first with them
+            if (!STUBS && !uair)
+                code = ldc(a.code, mo);
 #pragma unroll
             for (int p = 0; p < 12; ++p)
                 v[p] = ldo(Ak + p * 256, lo);
+            if (STUBS && !uair)
+                code = ldc(a.code, mo);
+            const unsigned m = code & 0xFFu;
+            pec = (m == 1u);
             if (ENERGY)
             {
 #pragma unroll
@@ -466,13 +474,13 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
             unsigned code = 0;
             if (act)
             {
-                code = ldc(a.code, mo);
-                const unsigned m = code & 0xFFu;
-                pec = (m == 1u);
                 float v[12];
 #pragma unroll
                 for (int q = 0; q < 6; ++q)
                     ldpair<PF, H>(A[q], bo, s, v[2 * q], v[2 * q + 1]);
+                code = ldc(a.code, mo);
+                const unsigned m = code & 0xFFu;
+                pec = (m == 1u);
                 if (ENERGY)
                 {
 #pragma unroll
