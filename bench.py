@@ -183,7 +183,7 @@ def served_run(count=200, rate=50.0, baseline_sample=20, seed=5):
     plain FP32 contexts."""
     from emsolver_amd import serve, tlm
     reqs = serve.served_stream(count=count, rate=rate, seed=seed)
-    srv = serve.TlmServer(block_ticks=32, autotune_ticks=2, keep_results=False)
+    srv = serve.TlmServer(block_ticks=32, autotune_ticks=2, keep_results=False, progress_s=20.0)
     done = srv.serve(reqs)
     summ = serve.summarize(done)
     summ.update({"rate_per_s": rate, "mean_members": round(srv.stats["mean_members"], 2),
@@ -213,6 +213,77 @@ def served_run(count=200, rate=50.0, baseline_sample=20, seed=5):
         f"{summ['gcell_s']:.2f} Gcell/s aggregate (mean {summ['mean_members']} members); "
         f"one at a time {summ['one_at_a_time_fp32_gcell_s']:.2f} Gcell/s")
     return summ
+
+
+def crop_scene(sc, lo, n):
+    """The n^3 subdomain [lo, lo+n)^3 of a cubic scene: materials and sources
+    copied, probes re-seeded inside (matched outer faces on the crop)."""
+    from emsolver_amd import scenes
+    m = sc.mat.reshape(sc.nz, sc.ny, sc.nx)[lo:lo + n, lo:lo + n, lo:lo + n].copy().reshape(-1)
+    out = scenes.Scene(n, n, n, m, sc.epsr, sc.sigma, total_steps=sc.total_steps,
+                       name=f"{sc.name}_crop{n}")
+    for c, p, a in sc.sources:
+        i, j, k = sc.coords(c)
+        if all(lo <= x < lo + n for x in (i, j, k)):
+            out.sources.append((out.cell(i - lo, j - lo, k - lo), p, a))
+    rng = np.random.default_rng(11)
+    probes = []
+    while len(probes) < len(sc.probes):
+        c = out.cell(*(int(x) for x in rng.integers(1, n - 1, 3)))
+        if m[c] != scenes.MAT_PEC and c not in probes:
+            probes.append(c)
+    out.probes = np.array(probes, np.int64)
+    return out
+
+
+def precision_report(steps=4000, crop=128):
+    """Error of the 16-bit pulse storage against the FP64 oracle: the 512^3
+    64-box scene cropped to its central crop^3 subdomain (source inside), all
+    16 probes, `steps` steps; and a dipole over the dielectric background for
+    the port (S-parameter) records. Max relative error per probe / port:
+    max_t |x - x64| / max_t |x64|, and the relative error of the input
+    impedance at f0 = 1 GHz."""
+    import dataclasses
+    from emsolver_amd import scenes, tlm
+    from tests import helpers
+    big = scenes.dielectric_boxes(n=512, nboxes=64, steps=steps, nprobes=16, nfreq=0, seed=1)
+    sc = crop_scene(big, 256 - crop // 2, crop)
+    t = time.perf_counter()
+    r64 = helpers.oracle_run(sc, fp64=True)
+    log(f"[bench] precision: FP64 oracle {crop}^3 x {steps} in {time.perf_counter() - t:.1f}s")
+    rep = {"scene": f"diel_boxes_512 (64 boxes) cropped to the central {crop}^3, {steps} steps, "
+                    f"16 probes (E, 3 components)", "probes": {}}
+    for pulse in ("f32", "f16", "bf16"):
+        g = tlm.run_scene(dataclasses.replace(sc, pulse=pulse), state=False)
+        e = np.abs(g.probes - r64.probes).max(axis=(0, 2)) / np.abs(r64.probes).max(axis=(0, 2))
+        rep["probes"][pulse] = {"max": float(e.max()), "median": float(np.median(e)),
+                                "per_probe": [float(x) for x in e]}
+        log(f"[bench] precision {pulse}: probe max rel err max {e.max():.2e} median {np.median(e):.2e}")
+    dip = scenes.dipole_scene(96, 96, 96, length_frac=0.5, steps=2000, background_boxes=8, seed=7)
+    dip.faces = False
+    dip.probes = np.array([dip.cell(20, 30, 40), dip.cell(70, 60, 50), dip.cell(48, 20, 48)], np.int64)
+    d64 = helpers.oracle_run(dip, fp64=True)
+    dt = float(tlm.timestep(dip.dl))
+
+    def zin(ports):
+        n = np.arange(ports.shape[0])
+        w = np.exp(-2j * np.pi * 1e9 * n * dt)
+        return (ports[:, 0, 0] * w).sum() / (ports[:, 0, 1] * w).sum()
+
+    z64 = zin(d64.ports)
+    rep["dipole"] = {"scene": "96^3 centre-fed dipole over 8 dielectric boxes, 2000 steps",
+                     "z64_ohm": [float(z64.real), float(z64.imag)]}
+    for pulse in ("f32", "f16", "bf16"):
+        g = tlm.run_scene(dataclasses.replace(dip, pulse=pulse), state=False)
+        ve = np.abs(g.ports - d64.ports).max(axis=0)[0] / np.abs(d64.ports).max(axis=0)[0]
+        pe = np.abs(g.probes - d64.probes).max(axis=(0, 2)) / np.abs(d64.probes).max(axis=(0, 2))
+        z = zin(g.ports)
+        rep["dipole"][pulse] = {"port_v": float(ve[0]), "port_i": float(ve[1]),
+                                "probe_max": float(pe.max()),
+                                "zin_rel": float(abs(z - z64) / abs(z64))}
+        log(f"[bench] precision dipole {pulse}: port V {ve[0]:.2e} I {ve[1]:.2e} "
+            f"Zin {abs(z - z64) / abs(z64):.2e} probes {pe.max():.2e}")
+    return rep
 
 
 def cpu_baseline(scene, budget_s=15.0):
@@ -251,6 +322,10 @@ def main():
     ap.add_argument("--extra-steps", type=int, default=200,
                     help="timed steps of each extra 512^3 line")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched and served lines")
+    ap.add_argument("--precision-report", action="store_true",
+                    help="only report the 16-bit storage error against the FP64 oracle (JSON)")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only run the served stream (--serve-count requests) and print its JSON")
     ap.add_argument("--serve-count", type=int, default=200,
                     help="requests of the served line (the full stream is 2000)")
     args = ap.parse_args()
@@ -265,6 +340,12 @@ def main():
     if tlm.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.precision_report:
+        print(json.dumps(precision_report()), flush=True)
+        return
+    if args.serve_only:
+        print(json.dumps(served_run(args.serve_count)), flush=True)
+        return
     dev_name = tlm.device_name(0)
     log(f"[bench] device {dev_name}")
 

@@ -116,6 +116,8 @@ int launch_step(tlm_ctx *c, int n)
     a.Pp = c->Pp;
     a.P = c->P;
     a.PB = c->PB;
+    a.h0 = c->h0;
+    a.h1 = c->h1;
     a.strips = c->strips;
     a.zc = c->zc;
     a.nzc = c->nzc;
@@ -410,14 +412,23 @@ int materialize(tlm_ctx *c, int step0)
     const long long slab = c->nxy;
     std::vector<float> Q(nz, 0.0f);
     std::vector<char> hot(nz, 0);
+    int klo = nz, khi = -1;
     for (size_t s = 0; s < c->src_cells.size(); ++s)
     {
         const int k = (int)(c->src_cells[s] / slab);
         Q[k] = Q[k] + 0.5f * std::fabs(c->src_amp[s]); // list order, as the oracle adds them
-        if (c->halo >= 0)
-            for (int d = -c->halo; d <= c->halo; ++d)
-                if (k + d >= 0 && k + d < nz)
-                    hot[k + d] = 1;
+        klo = std::min(klo, k);
+        khi = std::max(khi, k);
+    }
+    // one band of FP32 slabs: halo below the lowest source slab to halo above
+    // the highest, so the step computes slab offsets without a table lookup
+    c->h0 = c->h1 = 0;
+    if (c->halo >= 0 && khi >= 0)
+    {
+        c->h0 = std::max(0, klo - c->halo);
+        c->h1 = std::min(nz - 1, khi + c->halo) + 1;
+        for (int k = c->h0; k < c->h1; ++k)
+            hot[k] = 1;
     }
     std::vector<uint2> sl(nz);
     unsigned long long off = 0;

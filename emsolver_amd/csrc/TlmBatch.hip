@@ -161,6 +161,8 @@ int rebuild(tlm_batch *b)
         a.Pp = c->Pp;
         a.P = c->P;
         a.PB = c->PB;
+        a.h0 = c->h0;
+        a.h1 = c->h1;
         a.strips = strips;
         a.zc = zc;
         a.nzc = nzc;

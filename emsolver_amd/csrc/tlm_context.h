@@ -89,6 +89,7 @@ struct tlm_ctx
     int base = 0;      // step whose scales live in buffer 0
     int next_step = 0; // next step a 16-bit run must start at
     int nhot = 0;      // FP32 slabs
+    int h0 = 0, h1 = 0; // the FP32 slabs are [h0, h1)
     std::string err = "no error";
     std::vector<void *> owned;
     tlm_batch *batch = nullptr; // set while the context is a member of a batch

@@ -85,6 +85,7 @@ struct StepArgs
     long long Pp;        // slab pitch: roundup(nx*ny, 64) cells
     long long P;         // cells per plane = Pp*nz (stub planes)
     long long PB;        // bytes per pulse plane
+    int h0, h1;          // 16-bit formats: slabs [h0, h1) are stored in FP32
     int strips;          // 64-cell strips per slab: Pp/64
     int zc, nzc;         // planes per task, number of z chunks
     int tasks;           // strips * nzc
@@ -384,10 +385,13 @@ struct Slab16
 };
 __device__ __forceinline__ Slab16 slab16(const StepArgs &a, int k, int f)
 {
-    const uint2 s = a.slab[k];
-    const bool h = __builtin_amdgcn_readfirstlane(s.y) != 0u; // one format per wave
+    // the FP32 slabs form the band [h0, h1): offset and format are arithmetic,
+    // so the pulse loads never wait for a table lookup
+    const int hot_before = min(max(k - a.h0, 0), a.h1 - a.h0);
+    const bool h = __builtin_amdgcn_readfirstlane((int)(k < a.h0 || k >= a.h1)) != 0; // per wave
     const unsigned es = h ? 4u : 8u;
-    return {s.x + (unsigned)f * es, es, h, a.dsc[k], a.esc[k]};
+    const unsigned off = (unsigned)a.Pp * 4u * (unsigned)(k + hot_before);
+    return {off + (unsigned)f * es, es, h, a.dsc[k], a.esc[k]};
 }
 template <int PF, bool H16>
 __device__ __forceinline__ void ldpair(const char *plane, unsigned off, const Slab16 &s, float &x,

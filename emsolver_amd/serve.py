@@ -56,7 +56,7 @@ class TlmServer:
     """One device, one batch, requests admitted by free HBM."""
 
     def __init__(self, device=0, block_ticks=32, hbm_reserve=4 << 30, max_members=1024,
-                 tuning=None, autotune_ticks=0, keep_results=True):
+                 tuning=None, autotune_ticks=0, keep_results=True, progress_s=0.0):
         self.device = device
         self.block_ticks = int(block_ticks)
         self.hbm_reserve = int(hbm_reserve)
@@ -64,6 +64,7 @@ class TlmServer:
         self.tuning = tuning
         self.autotune_ticks = int(autotune_ticks)
         self.keep_results = keep_results
+        self.progress_s = float(progress_s)  # > 0: a progress line on stderr this often
         self.stats = {}
 
     def _need_bytes(self, sc) -> int:
@@ -117,7 +118,13 @@ class TlmServer:
         tuned = False
         ticks = 0
         member_ticks = 0
+        last_report = time.perf_counter()
         while arrivals or queue or live:
+            if self.progress_s > 0 and time.perf_counter() - last_report >= self.progress_s:
+                last_report = time.perf_counter()
+                import sys
+                print(f"[serve] t={last_report - t0:.0f}s done={len(done)} live={len(live)} "
+                      f"queued={len(queue)} to_arrive={len(arrivals)}", file=sys.stderr, flush=True)
             t = now()
             while arrivals and arrivals[0].arrival <= t:
                 queue.append(arrivals.popleft())

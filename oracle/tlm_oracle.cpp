@@ -686,27 +686,30 @@ template <class T> struct Solver : SolverBase
 
     // Injection bound per slab: sum of 0.5*|amp| over the soft sources, then
     // the port cells, in list order.
-    // Also marks the FP32 slabs: within fp32Halo slabs of a source slab
-    // (the strong reactive near field of a soft source is where 16-bit
-    // rounding costs the most accuracy).
+    // Also marks the FP32 slabs: the band from fp32Halo slabs below the lowest
+    // source slab to fp32Halo slabs above the highest (the strong reactive
+    // near field of a soft source is where 16-bit rounding costs the most
+    // accuracy).
     void injectionBound()
     {
         Qinj.assign(nz, 0.0f);
         hot.assign(nz, 0);
         const size_t slab = (size_t)nx * ny;
+        int klo = nz, khi = -1;
         auto mark = [&](int64_t c, float amp) {
             const int k = (int)((size_t)c / slab);
             Qinj[k] = Qinj[k] + 0.5f * std::fabs(amp);
-            if (fp32Halo >= 0)
-                for (int d = -fp32Halo; d <= fp32Halo; ++d)
-                    if (k + d >= 0 && k + d < nz)
-                        hot[k + d] = 1;
+            klo = std::min(klo, k);
+            khi = std::max(khi, k);
         };
         for (const auto &s : sources)
             mark(s.cell, s.amp);
         for (const auto &p : ports)
             for (int64_t c : p.cells)
                 mark(c, p.amp);
+        if (fp32Halo >= 0 && khi >= 0)
+            for (int k = std::max(0, klo - fp32Halo); k <= std::min(nz - 1, khi + fp32Halo); ++k)
+                hot[k] = 1;
     }
 
     // First step after set-up / set_state: slab maxima of the FP32 state,
