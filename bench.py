@@ -175,7 +175,7 @@ def batched_run(count=256, sample_every=4):
     return out
 
 
-def served_run(count=200, rate=50.0, baseline_sample=20, seed=5):
+def served_run(count=200, rate=50.0, baseline_sample=20, seed=5, max_cells=int(2.5e8)):
     """The served mixed-size stream (Poisson arrivals at `rate`/s, axes
     log-uniform in [32,384], steps U[500,4000], FP16 storage for the requests
     that tolerate 1e-3, FP32 otherwise) through the resident serving loop.
@@ -183,11 +183,12 @@ def served_run(count=200, rate=50.0, baseline_sample=20, seed=5):
     plain FP32 contexts."""
     from emsolver_amd import serve, tlm
     reqs = serve.served_stream(count=count, rate=rate, seed=seed)
-    srv = serve.TlmServer(block_ticks=32, autotune_ticks=2, keep_results=False, progress_s=20.0)
+    srv = serve.TlmServer(block_ticks=32, autotune_ticks=2, keep_results=False, progress_s=20.0,
+                          max_cells=max_cells)
     done = srv.serve(reqs)
     summ = serve.summarize(done)
     summ.update({"rate_per_s": rate, "mean_members": round(srv.stats["mean_members"], 2),
-                 "tuning": srv.stats["tuning"]})
+                 "tuning": srv.stats["tuning"], "admission": f"FIFO, free HBM, <= {max_cells} cells in flight"})
     alone_work, alone_wall = 0, 0.0
     for r in reqs[:baseline_sample]:
         sc = r.build()

@@ -154,6 +154,8 @@ tlm::PulseView pulse_view(const tlm_ctx *c, int n)
     v.Pp = c->Pp;
     v.pf = c->pf;
     v.slab = c->slab;
+    v.h0 = c->h0;
+    v.h1 = c->h1;
     v.sc = nullptr;
     v.isc = nullptr;
     if (c->pf != TLM_PULSE_F32)
@@ -431,19 +433,16 @@ int materialize(tlm_ctx *c, int step0)
             hot[k] = 1;
     }
     std::vector<uint2> sl(nz);
-    unsigned long long off = 0;
     c->nhot = 0;
     for (int k = 0; k < nz; ++k)
     {
-        // pair planes: two ports per cell, 8 bytes in an FP32 slab, 4 in a 16-bit one
-        sl[k] = make_uint2((unsigned)off, hot[k] ? 0u : 1u);
-        off += (unsigned long long)c->Pp * (hot[k] ? 8u : 4u);
+        // 64-cell blocks of 6 pairs: 24 bytes per cell in a 16-bit slab, 48 in an FP32 one
+        sl[k] = make_uint2(0u, hot[k] ? 0u : 1u);
         c->nhot += hot[k];
     }
-    if (off > (1ULL << 32) - (1ULL << 20))
-        return fail(c, TLM_ERR_UNSUPPORTED,
-                    "16-bit storage: a pair plane exceeds 32-bit byte offsets (too many cells)");
-    c->PB = (long long)((off + 255) / 256 * 256);
+    if ((unsigned long long)c->Pp * 48ULL > (1ULL << 32) - (1ULL << 20))
+        return fail(c, TLM_ERR_UNSUPPORTED, "16-bit storage: a slab exceeds 32-bit byte offsets");
+    c->PB = (long long)tlm::slab16_base(c->Pp, nz, c->h0, c->h1); // bytes of one buffer
     // slab maxima of the uploaded state
     std::vector<float> M(nz, 0.0f);
     if (!c->pendV.empty())
@@ -473,7 +472,7 @@ int materialize(tlm_ctx *c, int step0)
     for (char **buf : {&c->A, &c->B})
     {
         dfree(c, *buf);
-        int rc = dalloc(c, buf, (size_t)c->PB * 6);
+        int rc = dalloc(c, buf, (size_t)c->PB);
         if (rc)
             return rc;
     }
@@ -489,8 +488,8 @@ int materialize(tlm_ctx *c, int step0)
         (rc = upload<int>(c, &c->flag, nullptr, 1)))
         return rc;
     HIPCHK(c, hipMemsetAsync(c->flag, 0, sizeof(int), c->stream));
-    HIPCHK(c, hipMemsetAsync(c->A, 0, (size_t)c->PB * 6, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->B, 0, (size_t)c->PB * 6, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->A, 0, (size_t)c->PB, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->B, 0, (size_t)c->PB, c->stream));
     c->base = step0;
     c->materialized = true;
     if (!c->pendV.empty())

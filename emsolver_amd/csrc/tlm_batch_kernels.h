@@ -62,6 +62,8 @@ __device__ __forceinline__ PulseView member_view(const BatchMember &m, int n)
     v.Pp = m.st.Pp;
     v.pf = m.pf;
     v.slab = m.st.slab;
+    v.h0 = m.st.h0;
+    v.h1 = m.st.h1;
     const int cur = (n - m.sbase) & 1;
     v.sc = m.pf != PF_F32 ? m.scl[cur] : nullptr;
     v.isc = m.pf != PF_F32 ? m.iscl[cur] : nullptr;

@@ -369,76 +369,81 @@ __device__ __forceinline__ void strip_task(const StepArgs &a, int task, int lane
 }
 
 // ---------------------------------------------------------------------------
-// 16-bit formats. The pulses live in 6 pair planes, plane q holding ports
-// 2q and 2q+1 (the two polarisations of one face) of every cell: a 16-bit
-// slab stores the pair as one dword (two codes), an FP32 slab as a float2.
-// Every store of the step writes both polarisations of one face of one cell
-// with the same value rule, so a lane moves one dword (or 8 bytes) per
-// face: 6 loads and 6 stores per cell, as many bytes per instruction as the
-// FP32 layout.
+// 16-bit formats. Pulses are grouped in 6 pairs, pair q holding ports 2q and
+// 2q+1 (the two polarisations of one face). A slab is a run of 64-cell
+// blocks, a block holding its cells' 6 pairs pair after pair: in a 16-bit
+// slab a pair is one dword (two codes, 1.5 KB per block), in an FP32 slab a
+// float2 (3 KB per block). The FP32 slabs form one band [h0, h1), so slab k
+// starts at Pp*24*(k + FP32 slabs below k) bytes. Every store of the step
+// writes both polarisations of one face of one cell with the same value
+// rule, so a lane moves one dword (or 8 bytes) per face.
 // ---------------------------------------------------------------------------
 struct Slab16
 {
-    unsigned bo, es; // byte offset of the lane's cell pair inside a plane, bytes per cell
+    size_t base;     // byte offset of the slab in the buffer
+    unsigned lo, es; // lane's cell offset inside the slab, bytes per cell and pair
     bool h16;
     float ds, isc;
 };
+// Byte offset of slab k in a 16-bit buffer (FP32 slabs [h0, h1) take twice
+// the bytes of a 16-bit slab).
+__host__ __device__ inline size_t slab16_base(long long Pp, long long k, int h0, int h1)
+{
+    const long long hot = k < h0 ? 0 : (k < h1 ? k - h0 : h1 - h0);
+    return (size_t)Pp * 24u * (size_t)(k + hot);
+}
+// Offset of in-slab cell g for es bytes per pair: 64-cell blocks of 6 pairs.
+__device__ __forceinline__ unsigned blk16(unsigned g, unsigned es)
+{
+    return (g >> 6) * (384u * es) + (g & 63u) * es;
+}
 __device__ __forceinline__ Slab16 slab16(const StepArgs &a, int k, int f)
 {
-    // the FP32 slabs form the band [h0, h1): offset and format are arithmetic,
-    // so the pulse loads never wait for a table lookup
-    const int hot_before = min(max(k - a.h0, 0), a.h1 - a.h0);
+    // offset and format are arithmetic, so the pulse loads never wait for a
+    // table lookup
     const bool h = __builtin_amdgcn_readfirstlane((int)(k < a.h0 || k >= a.h1)) != 0; // per wave
     const unsigned es = h ? 4u : 8u;
-    const unsigned off = (unsigned)a.Pp * 4u * (unsigned)(k + hot_before);
-    return {off + (unsigned)f * es, es, h, a.dsc[k], a.esc[k]};
+    const size_t base = slab16_base(a.Pp, k, a.h0, a.h1);
+    return {base, blk16((unsigned)f, es), es, h, a.dsc[k], a.esc[k]};
 }
 template <int PF, bool H16>
-__device__ __forceinline__ void ldpair(const char *plane, unsigned off, const Slab16 &s, float &x,
+__device__ __forceinline__ void ldpair(const char *p, unsigned off, const Slab16 &s, float &x,
                                        float &y)
 {
     if (H16)
     {
-        const unsigned u = *reinterpret_cast<const unsigned *>(plane + off);
+        const unsigned u = *reinterpret_cast<const unsigned *>(p + off);
         x = dec16<PF>(u & 0xffffu) * s.ds;
         y = dec16<PF>(u >> 16) * s.ds;
     }
     else
     {
-        const float2 w = *reinterpret_cast<const float2 *>(plane + off);
+        const float2 w = *reinterpret_cast<const float2 *>(p + off);
         x = w.x;
         y = w.y;
     }
 }
 template <int PF, bool H16>
-__device__ __forceinline__ void stpair(char *plane, unsigned off, const Slab16 &s, float x, float y)
+__device__ __forceinline__ void stpair(char *p, unsigned off, const Slab16 &s, float x, float y)
 {
     if (H16)
-        *reinterpret_cast<unsigned *>(plane + off) =
+        *reinterpret_cast<unsigned *>(p + off) =
             (unsigned)enc16<PF>(x * s.isc) | ((unsigned)enc16<PF>(y * s.isc) << 16);
     else
-        *reinterpret_cast<float2 *>(plane + off) = make_float2(x, y);
+        *reinterpret_cast<float2 *>(p + off) = make_float2(x, y);
 }
-// format known only at run time (another slab; the branch is wave-uniform)
+// pair q of another slab's cell (format known at run time, wave-uniform)
 template <int PF>
-__device__ __forceinline__ void ldpair_d(const char *plane, unsigned off, const Slab16 &s, float &x,
-                                         float &y)
+__device__ __forceinline__ void stpair_at(char *buf, const Slab16 &s, int q, float x, float y)
 {
+    char *p = buf + s.base + (size_t)q * 64u * s.es;
     if (s.h16)
-        ldpair<PF, true>(plane, off, s, x, y);
+        stpair<PF, true>(p, s.lo, s, x, y);
     else
-        ldpair<PF, false>(plane, off, s, x, y);
-}
-template <int PF>
-__device__ __forceinline__ void stpair_d(char *plane, unsigned off, const Slab16 &s, float x, float y)
-{
-    if (s.h16)
-        stpair<PF, true>(plane, off, s, x, y);
-    else
-        stpair<PF, false>(plane, off, s, x, y);
+        stpair<PF, false>(p, s.lo, s, x, y);
 }
 
-// The fused step of strip_task with pair planes and per-slab formats (same
+// The fused step of strip_task with pair blocks and per-slab formats (same
 // delivery rules, same FP32 arithmetic, values rounded where stored).
 template <bool STUBS, bool ENERGY, int PF>
 __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int lane, double &esum)
@@ -449,29 +454,24 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
     const int j = f / a.nx, i = f - j * a.nx;
     const int k0 = tz * a.zc, k1 = min(a.nz, k0 + a.zc);
     const float rho = a.rho;
-    const char *A[6];
-    char *B[6];
-#pragma unroll
-    for (int q = 0; q < 6; ++q)
-    {
-        A[q] = a.A + q * a.PB;
-        B[q] = a.B + q * a.PB;
-    }
     const bool hasL = act && lane > 0 && i > 0;
     const bool hasR = act && lane < kWave - 1 && i < a.nx - 1;
     const bool bx0 = i == 0, bx1 = i == a.nx - 1, by0 = j == 0, by1 = j == a.ny - 1;
     float zpo0 = 0.f, zpo1 = 0.f;
     bool pecPrev = false;
     unsigned mo = (unsigned)(k0 * a.Pp + f);
-    Slab16 sp = {0u, 8u, false, 1.0f, 1.0f};
+    Slab16 sp = {0, 0u, 8u, false, 1.0f, 1.0f};
 
     for (int k = k0; k < k1; ++k, mo += (unsigned)a.Pp)
     {
         const Slab16 s = slab16(a, k, f);
         auto body = [&](auto h16_tag) {
             constexpr bool H = decltype(h16_tag)::value;
-            const unsigned bo = s.bo;
-            const unsigned by = (unsigned)a.nx * s.es;
+            constexpr unsigned ES = H ? 4u : 8u;
+            constexpr unsigned PQ = 64u * ES; // bytes between pairs of a block
+            const char *Ak = a.A + s.base;
+            char *Bk = a.B + s.base;
+            const unsigned bo = s.lo;
             float o[12];
             float mc = 0.f;
             bool pec = true;
@@ -481,7 +481,7 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
                 float v[12];
 #pragma unroll
                 for (int q = 0; q < 6; ++q)
-                    ldpair<PF, H>(A[q], bo, s, v[2 * q], v[2 * q + 1]);
+                    ldpair<PF, H>(Ak + q * PQ, bo, s, v[2 * q], v[2 * q + 1]);
                 code = ldc(a.code, mo);
                 const unsigned m = code & 0xFFu;
                 pec = (m == 1u);
@@ -530,93 +530,88 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
                 // ---- x faces: pair 0 (x-: ports 0,1), pair 1 (x+: ports 2,3) ----
                 const bool pecL = (code & NB_XM) != 0u, pecR = (code & NB_XP) != 0u;
                 if (pec)
-                    stpair<PF, H>(B[0], bo, s, bx0 ? rho * o[0] : o[0], bx0 ? rho * o[1] : o[1]);
+                    stpair<PF, H>(Bk, bo, s, bx0 ? rho * o[0] : o[0], bx0 ? rho * o[1] : o[1]);
                 else if (bx0)
-                    stpair<PF, H>(B[0], bo, s, rho * o[0], rho * o[1]);
+                    stpair<PF, H>(Bk, bo, s, rho * o[0], rho * o[1]);
                 else if (pecL)
-                    stpair<PF, H>(B[0], bo, s, -o[0], -o[1]);
+                    stpair<PF, H>(Bk, bo, s, -o[0], -o[1]);
                 else if (hasL)
-                    stpair<PF, H>(B[0], bo, s, L0, L1);
+                    stpair<PF, H>(Bk, bo, s, L0, L1);
                 else
-                    stpair<PF, H>(B[1], bo - s.es, s, o[0], o[1]);
+                    stpair<PF, H>(Bk + PQ, blk16((unsigned)f - 1u, ES), s, o[0], o[1]);
                 if (pec)
-                    stpair<PF, H>(B[1], bo, s, bx1 ? rho * o[2] : o[2], bx1 ? rho * o[3] : o[3]);
+                    stpair<PF, H>(Bk + PQ, bo, s, bx1 ? rho * o[2] : o[2], bx1 ? rho * o[3] : o[3]);
                 else if (bx1)
-                    stpair<PF, H>(B[1], bo, s, rho * o[2], rho * o[3]);
+                    stpair<PF, H>(Bk + PQ, bo, s, rho * o[2], rho * o[3]);
                 else if (pecR)
-                    stpair<PF, H>(B[1], bo, s, -o[2], -o[3]);
+                    stpair<PF, H>(Bk + PQ, bo, s, -o[2], -o[3]);
                 else if (hasR)
-                    stpair<PF, H>(B[1], bo, s, R0, R1);
+                    stpair<PF, H>(Bk + PQ, bo, s, R0, R1);
                 else
-                    stpair<PF, H>(B[0], bo + s.es, s, o[2], o[3]);
+                    stpair<PF, H>(Bk, blk16((unsigned)f + 1u, ES), s, o[2], o[3]);
                 // ---- y faces: pair 2 (y-: ports 4,5), pair 3 (y+: ports 6,7) ----
                 const bool pecLo = (code & NB_YM) != 0u, pecHi = (code & NB_YP) != 0u;
                 if (pec)
                 {
-                    stpair<PF, H>(B[2], bo, s, by0 ? rho * o[4] : o[4], by0 ? rho * o[5] : o[5]);
-                    stpair<PF, H>(B[3], bo, s, by1 ? rho * o[6] : o[6], by1 ? rho * o[7] : o[7]);
+                    stpair<PF, H>(Bk + 2 * PQ, bo, s, by0 ? rho * o[4] : o[4], by0 ? rho * o[5] : o[5]);
+                    stpair<PF, H>(Bk + 3 * PQ, bo, s, by1 ? rho * o[6] : o[6], by1 ? rho * o[7] : o[7]);
                 }
                 else
                 {
                     if (by0)
-                        stpair<PF, H>(B[2], bo, s, rho * o[4], rho * o[5]);
+                        stpair<PF, H>(Bk + 2 * PQ, bo, s, rho * o[4], rho * o[5]);
                     else if (pecLo)
-                        stpair<PF, H>(B[2], bo, s, -o[4], -o[5]);
+                        stpair<PF, H>(Bk + 2 * PQ, bo, s, -o[4], -o[5]);
                     else
-                        stpair<PF, H>(B[3], bo - by, s, o[4], o[5]);
+                        stpair<PF, H>(Bk + 3 * PQ, blk16((unsigned)(f - a.nx), ES), s, o[4], o[5]);
                     if (by1)
-                        stpair<PF, H>(B[3], bo, s, rho * o[6], rho * o[7]);
+                        stpair<PF, H>(Bk + 3 * PQ, bo, s, rho * o[6], rho * o[7]);
                     else if (pecHi)
-                        stpair<PF, H>(B[3], bo, s, -o[6], -o[7]);
+                        stpair<PF, H>(Bk + 3 * PQ, bo, s, -o[6], -o[7]);
                     else
-                        stpair<PF, H>(B[2], bo + by, s, o[6], o[7]);
+                        stpair<PF, H>(Bk + 2 * PQ, blk16((unsigned)(f + a.nx), ES), s, o[6], o[7]);
                 }
                 // ---- z faces: pair 4 (z-: ports 8,9), pair 5 (z+: ports 10,11) ----
                 if (k > k0)
                 {
                     // pull: slab k-1's reflected z+ pulses are in registers
                     if (pec)
-                        stpair<PF, H>(B[4], bo, s, o[8], o[9]);
+                        stpair<PF, H>(Bk + 4 * PQ, bo, s, o[8], o[9]);
                     else
-                        stpair<PF, H>(B[4], bo, s, pecPrev ? -o[8] : zpo0, pecPrev ? -o[9] : zpo1);
+                        stpair<PF, H>(Bk + 4 * PQ, bo, s, pecPrev ? -o[8] : zpo0,
+                                      pecPrev ? -o[9] : zpo1);
                     // slab k-1's z+ slot: its kept pulses if it is PEC (then
                     // zpo holds them), else what this cell sends down
                     if (pecPrev)
-                        stpair_d<PF>(B[5], sp.bo, sp, zpo0, zpo1);
+                        stpair_at<PF>(a.B, sp, 5, zpo0, zpo1);
                     else
-                        stpair_d<PF>(B[5], sp.bo, sp, pec ? -zpo0 : o[8], pec ? -zpo1 : o[9]);
+                        stpair_at<PF>(a.B, sp, 5, pec ? -zpo0 : o[8], pec ? -zpo1 : o[9]);
                 }
                 else if (k == 0)
-                    stpair<PF, H>(B[4], bo, s, rho * o[8], rho * o[9]);
+                    stpair<PF, H>(Bk + 4 * PQ, bo, s, rho * o[8], rho * o[9]);
                 else
                 {
                     const bool pecZ = (code & NB_ZM) != 0u;
                     if (pec)
-                        stpair<PF, H>(B[4], bo, s, o[8], o[9]);
+                        stpair<PF, H>(Bk + 4 * PQ, bo, s, o[8], o[9]);
                     else if (pecZ)
-                        stpair<PF, H>(B[4], bo, s, -o[8], -o[9]);
+                        stpair<PF, H>(Bk + 4 * PQ, bo, s, -o[8], -o[9]);
                     else
-                    {
-                        const Slab16 sm = slab16(a, k - 1, f);
-                        stpair_d<PF>(B[5], sm.bo, sm, o[8], o[9]);
-                    }
+                        stpair_at<PF>(a.B, slab16(a, k - 1, f), 5, o[8], o[9]);
                 }
                 if (k == k1 - 1)
                 {
                     if (k == a.nz - 1)
-                        stpair<PF, H>(B[5], bo, s, rho * o[10], rho * o[11]);
+                        stpair<PF, H>(Bk + 5 * PQ, bo, s, rho * o[10], rho * o[11]);
                     else
                     {
                         const bool pecZ = (code & NB_ZP) != 0u;
                         if (pec)
-                            stpair<PF, H>(B[5], bo, s, o[10], o[11]);
+                            stpair<PF, H>(Bk + 5 * PQ, bo, s, o[10], o[11]);
                         else if (pecZ)
-                            stpair<PF, H>(B[5], bo, s, -o[10], -o[11]);
+                            stpair<PF, H>(Bk + 5 * PQ, bo, s, -o[10], -o[11]);
                         else
-                        {
-                            const Slab16 sq = slab16(a, k + 1, f);
-                            stpair_d<PF>(B[4], sq.bo, sq, o[10], o[11]);
-                        }
+                            stpair_at<PF>(a.B, slab16(a, k + 1, f), 4, o[10], o[11]);
                     }
                 }
             }
@@ -692,49 +687,48 @@ struct PulseView
     long long PB;     // bytes per plane
     long long Pp;     // slab pitch (cells)
     int pf;           // PF_*
-    const uint2 *slab; // 16-bit formats: per slab (byte offset, 16-bit flag)
+    const uint2 *slab; // 16-bit formats: per slab (unused, 16-bit flag)
+    int h0, h1;        // 16-bit formats: the FP32 slabs [h0, h1)
     const float *sc;  // per slab 2^E (current step)
     const float *isc; // per slab 2^-E
 };
-// FP32 contexts: 64-cell blocks of 12 ports per slab. 16-bit contexts: 6 pair planes
-// (plane p/2, half p&1), a 16-bit slab holding two codes per cell, an FP32
-// slab a float2.
+// FP32 contexts: 64-cell blocks of 12 ports per slab. 16-bit contexts: 64-cell
+// blocks of 6 pairs (pair p/2, half p&1), two codes per cell and pair in a
+// 16-bit slab, a float2 in an FP32 slab.
 __device__ __forceinline__ float pv_get(const PulseView &v, int p, long long c)
 {
+    const long long k = c / v.Pp, f = c - k * v.Pp;
     if (v.pf == PF_F32)
-    {
-        const long long k = c / v.Pp, f = c - k * v.Pp;
         return *reinterpret_cast<const float *>(v.base + k * v.Pp * 48 + p * 256 +
                                                 blk_off((unsigned)f));
-    }
-    const char *plane = v.base + (p >> 1) * v.PB;
-    const long long k = c / v.Pp, f = c - k * v.Pp;
     const uint2 s = v.slab[k];
+    const unsigned es = s.y ? 4u : 8u;
+    const char *q = v.base + slab16_base(v.Pp, k, v.h0, v.h1) + (size_t)(p >> 1) * 64u * es + blk16((unsigned)f, es);
     if (!s.y)
-        return reinterpret_cast<const float *>(plane + s.x)[2 * f + (p & 1)];
-    const unsigned h = reinterpret_cast<const unsigned short *>(plane + s.x)[2 * f + (p & 1)];
+        return reinterpret_cast<const float *>(q)[p & 1];
+    const unsigned h = reinterpret_cast<const unsigned short *>(q)[p & 1];
     return (v.pf == PF_F16 ? dec16<PF_F16>(h) : dec16<PF_BF16>(h)) * v.sc[k];
 }
 // Stores x; returns |x * 2^-E| for a 16-bit slot (0 for FP32) so callers can
 // check the range.
 __device__ __forceinline__ float pv_set(const PulseView &v, int p, long long c, float x)
 {
+    const long long k = c / v.Pp, f = c - k * v.Pp;
     if (v.pf == PF_F32)
     {
-        const long long k = c / v.Pp, f = c - k * v.Pp;
         *reinterpret_cast<float *>(v.base + k * v.Pp * 48 + p * 256 + blk_off((unsigned)f)) = x;
         return 0.f;
     }
-    char *plane = v.base + (p >> 1) * v.PB;
-    const long long k = c / v.Pp, f = c - k * v.Pp;
     const uint2 s = v.slab[k];
+    const unsigned es = s.y ? 4u : 8u;
+    char *q = v.base + slab16_base(v.Pp, k, v.h0, v.h1) + (size_t)(p >> 1) * 64u * es + blk16((unsigned)f, es);
     if (!s.y)
     {
-        reinterpret_cast<float *>(plane + s.x)[2 * f + (p & 1)] = x;
+        reinterpret_cast<float *>(q)[p & 1] = x;
         return 0.f;
     }
     const float y = x * v.isc[k];
-    reinterpret_cast<unsigned short *>(plane + s.x)[2 * f + (p & 1)] =
+    reinterpret_cast<unsigned short *>(q)[p & 1] =
         v.pf == PF_F16 ? enc16<PF_F16>(y) : enc16<PF_BF16>(y);
     return fabsf(y);
 }

@@ -56,7 +56,8 @@ class TlmServer:
     """One device, one batch, requests admitted by free HBM."""
 
     def __init__(self, device=0, block_ticks=32, hbm_reserve=4 << 30, max_members=1024,
-                 tuning=None, autotune_ticks=0, keep_results=True, progress_s=0.0):
+                 tuning=None, autotune_ticks=0, keep_results=True, progress_s=0.0,
+                 max_cells=0):
         self.device = device
         self.block_ticks = int(block_ticks)
         self.hbm_reserve = int(hbm_reserve)
@@ -65,6 +66,10 @@ class TlmServer:
         self.autotune_ticks = int(autotune_ticks)
         self.keep_results = keep_results
         self.progress_s = float(progress_s)  # > 0: a progress line on stderr this often
+        # > 0: admit while the cells in flight stay below this (first come,
+        # first served; enough cells to keep HBM streaming, few enough that a
+        # request's ticks stay short)
+        self.max_cells = int(max_cells)
         self.stats = {}
 
     def _need_bytes(self, sc) -> int:
@@ -134,7 +139,9 @@ class TlmServer:
                 sc = take(r.rid)
                 need = self._need_bytes(sc)
                 free, _ = tlm.device_memory(self.device)
-                if live and need + self.hbm_reserve > free:
+                cells = sum(v[1].ncell for v in live.values())
+                over = self.max_cells > 0 and cells + sc.ncell > self.max_cells
+                if live and (need + self.hbm_reserve > free or over):
                     with cond:  # put it back; try again after members leave
                         ready[r.rid] = sc
                         cond.notify_all()
