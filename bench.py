@@ -112,6 +112,9 @@ def gpu_run(scene, warmup, steps, tag):
     return out
 
 
+GRAPH_TICKS = 32  # ticks per captured HIP graph in the batched and served lines
+
+
 def batched_run(count=256, sample_every=4):
     """256 independent dipole grids (axes log-uniform in [32,128], steps
     U[500,2000]) in one batch: autotuned launch shape, graph-captured blocks
@@ -125,6 +128,7 @@ def batched_run(count=256, sample_every=4):
     log(f"[bench] batched_{count}: {sum(sc.ncell for sc in scs) / 1e6:.1f} M cells, "
         f"{work / 1e9:.1f} G cell-updates, set-up {time.perf_counter() - t:.1f}s")
     with tlm.TlmBatch() as b:
+        b.set_tuning(graph_ticks=GRAPH_TICKS)
         for d, sc in zip(devs, scs):
             b.add(d, sc.total_steps)
         t0 = time.perf_counter()
@@ -184,7 +188,7 @@ def served_run(count=200, rate=50.0, baseline_sample=20, seed=5, max_cells=int(2
     from emsolver_amd import serve, tlm
     reqs = serve.served_stream(count=count, rate=rate, seed=seed)
     srv = serve.TlmServer(block_ticks=32, autotune_ticks=2, keep_results=False, progress_s=20.0,
-                          max_cells=max_cells)
+                          max_cells=max_cells, tuning={"graph_ticks": GRAPH_TICKS})
     done = srv.serve(reqs)
     summ = serve.summarize(done)
     summ.update({"rate_per_s": rate, "mean_members": round(srv.stats["mean_members"], 2),
@@ -323,6 +327,12 @@ def main():
     ap.add_argument("--extra-steps", type=int, default=200,
                     help="timed steps of each extra 512^3 line")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched and served lines")
+    ap.add_argument("--lines", default="",
+                    help="comma list of extra lines to run (diel_boxes_512_f32, diel_boxes_512_f16, "
+                         "diel_boxes_512_bf16, free_space_512_f32, batched_256, served); default all")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="plain launches instead of captured HIP graphs in the batched lines "
+                         "(for tracing tools that do not follow graph launches)")
     ap.add_argument("--precision-report", action="store_true",
                     help="only report the 16-bit storage error against the FP64 oracle (JSON)")
     ap.add_argument("--serve-only", action="store_true",
@@ -341,6 +351,9 @@ def main():
     if tlm.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    global GRAPH_TICKS
+    if args.no_graph:
+        GRAPH_TICKS = 0
     if args.precision_report:
         print(json.dumps(precision_report()), flush=True)
         return
@@ -367,8 +380,13 @@ def main():
     if lib.tlm_copy_bandwidth(0, 2 << 30, 20, C.byref(gbps), C.byref(msc)) == 0:
         extra["device_copy_gbs"] = round(gbps.value, 1)
         log(f"[bench] device copy {gbps.value:.0f} GB/s")
+    lines = set(args.lines.split(",")) if args.lines else None
+
+    def want(name):
+        return lines is None or name in lines
+
     if not args.no_extra:
-        for pulse in ("f32", "f16", "bf16"):
+        for pulse in [p for p in ("f32", "f16", "bf16") if want(f"diel_boxes_512_{p}")]:
             sc16 = scenes.dielectric_boxes(n=512, nboxes=64, steps=args.extra_steps + 10,
                                            nprobes=16, nfreq=32, seed=1)
             sc16.pulse = pulse
@@ -385,6 +403,7 @@ def main():
             log(f"[bench] diel_boxes_512_{pulse}: {r['gcells']:.2f} Gcell/s, kernel "
                 f"{r['kernel_ms_avg']:.3f} ms, {r['achieved_gbs']:.0f} GB/s = "
                 f"{r['achieved_gbs'] / HBM_PEAK_GBS:.3f} of peak")
+    if not args.no_extra and want("free_space_512_f32"):
         big = scenes.free_space_cube(n=512, steps=args.extra_steps + 10, probe_offset=10)
         r = gpu_run(big, 10, args.extra_steps, "free_space_512")
         extra["free_space_512_f32"] = {
@@ -398,8 +417,9 @@ def main():
         log(f"[bench] free_space_512: {r['gcells']:.2f} Gcell/s, kernel {r['kernel_ms_avg']:.3f} ms, "
             f"{r['achieved_gbs']:.0f} GB/s = {r['achieved_gbs'] / HBM_PEAK_GBS:.3f} of peak")
 
-    if not args.no_batch:
+    if not args.no_batch and want("batched_256"):
         extra["batched_256"] = batched_run(256)
+    if not args.no_batch and want("served"):
         extra[f"served_{args.serve_count}"] = served_run(args.serve_count)
 
     cpu = None
