@@ -33,6 +33,10 @@ class BatchTuning(C.Structure):
                 ("graph_ticks", C.c_int32), ("planes16", C.c_int32)]
 
 
+class VoxelGrid(C.Structure):
+    _fields_ = [("origin", C.c_float * 3), ("dl", C.c_float)]
+
+
 class Tuning(C.Structure):
     _fields_ = [("kernel", C.c_int32), ("planes", C.c_int32), ("block_waves", C.c_int32),
                 ("nontemporal", C.c_int32), ("grid_blocks", C.c_int32)]
@@ -98,6 +102,20 @@ SIGNATURES = {
     "tlm_estimate_bytes": (C.c_int64, [C.POINTER(GridDesc), C.c_int]),
     "tlm_device_encode": (C.c_int, [C.c_int, C.c_int, fp, C.c_float, C.POINTER(C.c_uint16),
                                     C.c_int64]),
+    "tlm_mat_fill": (C.c_int, [P, C.c_uint8]),
+    "tlm_mat_cells_box": (C.c_int, [P, i32p, i32p, C.c_uint8]),
+    "tlm_voxelize_mesh": (C.c_int, [P, C.POINTER(VoxelGrid), fp, C.c_int32, i32p, C.c_int32, fp,
+                                    C.c_uint8, C.c_int]),
+    "tlm_voxelize_wire": (C.c_int, [P, C.POINTER(VoxelGrid), fp, C.c_int32, C.c_float, fp,
+                                    C.c_uint8]),
+    "tlm_carve_feed": (C.c_int, [P, C.POINTER(VoxelGrid), fp, fp, fp, i64p, C.c_int32, i32p]),
+    "tlm_mat_read": (C.c_int, [P, u8p]),
+    "tlm_mat_write": (C.c_int, [P, u8p]),
+    "tlm_upload_mat": (C.c_int, [P, fp, fp, C.c_int]),
+    "tlm_pec_faces_mat": (C.c_int64, [P]),
+    "tlm_huygens_mat": (C.c_int64, [P, C.c_int]),
+    "tlm_read_faces": (C.c_int64, [P, i64p, i32p, C.c_int64]),
+    "tlm_read_huygens_faces": (C.c_int64, [P, i64p, i32p, C.c_int64]),
 }
 
 

@@ -694,6 +694,17 @@ int tlm_upload(tlm_ctx *c, const uint8_t *mat, const float *yhat, const float *g
         if (rc0)
             return rc0;
     }
+    return install_material(c, any_diel, yhat, ghat, nmat, V, Vs);
+}
+
+} // extern "C"
+
+namespace tlmi {
+
+int install_material(tlm_ctx *c, bool any_diel, const float *yhat, const float *ghat, int nmat,
+                     const float *V, const float *Vs)
+{
+    const long long ncell = (long long)c->nx * c->ny * c->nz;
     // per-material (yh, gh, den), den = 4 + yh + gh as in TlmSolver.cpp:90
     std::vector<float4> props(std::max(1, nmat));
     for (int i = 0; i < nmat; ++i)
@@ -747,6 +758,10 @@ int tlm_upload(tlm_ctx *c, const uint8_t *mat, const float *yhat, const float *g
     c->dft_done = 0;
     return TLM_OK;
 }
+
+} // namespace tlmi
+
+extern "C" {
 
 int tlm_set_sources(tlm_ctx *c, const int64_t *cells, const int32_t *pol, const float *amp, int n)
 {

@@ -7,8 +7,11 @@
 // every monitor write are the single-grid code paths, only the launch that
 // drives them differs. A tick advances every active member by one step (see
 // tlm_batch_kernels.h); blocks of ticks are captured in a HIP graph.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wunused-function" // single-grid kernels of the shared header
 #include "tlm_batch_kernels.h"
 #include "tlm_context.h"
+#pragma clang diagnostic pop
 
 #include <hip/hip_runtime.h>
 

@@ -93,6 +93,7 @@ struct tlm_ctx
     std::string err = "no error";
     std::vector<void *> owned;
     tlm_batch *batch = nullptr; // set while the context is a member of a batch
+    uint8_t *mat = nullptr;     // device material map (cellIndex order), built on demand
 };
 
 namespace tlmi {
@@ -180,6 +181,10 @@ inline float format_max(int pf)
 tlm::PulseView pulse_view(const tlm_ctx *c, int n);
 // 16-bit formats: lay out the slabs and store the uploaded state (first run).
 int materialize(tlm_ctx *c, int step0);
+// Second half of an upload, once the cell codes and block flags are on the
+// device: material table, zeroed or uploaded state, stub buffer.
+int install_material(tlm_ctx *c, bool any_diel, const float *yhat, const float *ghat, int nmat,
+                     const float *V, const float *Vs);
 // Running FP64 probe DFT over steps [n0, n1) (queued on stream s).
 void queue_probe_dft(tlm_ctx *c, int n0, int n1, hipStream_t s);
 

@@ -73,8 +73,9 @@ class TlmServer:
         self.stats = {}
 
     def _need_bytes(self, sc) -> int:
-        stubs = bool((sc.mat >= scenes.MAT_DIEL0).any())
-        b = tlm.estimate_bytes(sc.nx, sc.ny, sc.nz, sc.pulse, stubs)
+        b = tlm.estimate_bytes(sc.nx, sc.ny, sc.nz, sc.pulse, sc.has_diel)
+        if sc.ops is not None:
+            b += sc.ncell  # the device material map of a device-built scene
         if sc.pulse != "f32" and sc.fp32_halo >= 0:
             # FP32 slabs near the sources take 8 bytes per pair instead of 4
             b += int(2 * 12 * sc.nx * sc.ny * min(sc.nz, 2 * sc.fp32_halo + 2) * 2)
@@ -202,12 +203,14 @@ def summarize(done, wall_s=None) -> dict:
 
 
 def served_stream(count=2000, rate=50.0, lo=32, hi=384, steps=(500, 4000), seed=5,
-                  f16_share=0.5, nprobes=2):
+                  f16_share=0.5, nprobes=2, device_setup=False):
     """The served workload: Poisson arrivals at `rate`/s, each axis log-uniform
     in [lo, hi], the dielectric-box background with a centre-fed dipole of
     seeded length, steps uniform in [steps], per-request tolerance 1e-3
     (FP16 storage) or 1e-5 (FP32) with a seeded split. Scenes are built on
-    demand (Request.build)."""
+    demand (Request.build): as host material maps, or with device_setup as
+    set-up operations that the device voxelises at admission (the same
+    grids)."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -220,24 +223,28 @@ def served_stream(count=2000, rate=50.0, lo=32, hi=384, steps=(500, 4000), seed=
         sseed = int(rng.integers(1 << 31))
 
         def build(dims=dims, frac=frac, st=st, tol=tol, sseed=sseed, rid=rid):
-            return request_scene(dims, frac, st, storage_for(tol), sseed, nprobes, f"req{rid}")
+            return request_scene(dims, frac, st, storage_for(tol), sseed, nprobes, f"req{rid}",
+                                 device_setup)
 
         out.append(Request(rid, None, t, tol, build))
     return out
 
 
-def request_scene(dims, length_frac, steps, pulse, seed, nprobes=2, name="req"):
+def request_scene(dims, length_frac, steps, pulse, seed, nprobes=2, name="req",
+                  device_setup=False):
     nx, ny, nz = dims
     sc = scenes.dipole_scene(nx, ny, nz, length_frac=length_frac, steps=steps,
-                             background_boxes=8, seed=seed)
+                             background_boxes=8, seed=seed, device_setup=device_setup)
     sc.faces = False
     sc.pulse = pulse
     rng = np.random.default_rng(seed + 1)
     probes = []
     while len(probes) < nprobes:
-        c = sc.cell(int(rng.integers(1, nx - 1)), int(rng.integers(1, ny - 1)),
-                    int(rng.integers(1, nz - 1)))
-        if sc.mat[c] != scenes.MAT_PEC and c not in probes:
+        i, j, k = (int(rng.integers(1, nx - 1)), int(rng.integers(1, ny - 1)),
+                   int(rng.integers(1, nz - 1)))
+        c = sc.cell(i, j, k)
+        # off the dipole's column (its only PEC cells)
+        if (i, j) != (nx // 2, ny // 2) and c not in probes:
             probes.append(c)
     sc.probes = np.array(probes, np.int64)
     sc.name = name

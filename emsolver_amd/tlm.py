@@ -168,6 +168,107 @@ class TlmDevice:
                                       ptr(Vsa, C.c_float)), "tlm_upload")
         self.has_stubs = bool((mat >= 2).any())
 
+    # ---- voxel set-up on the device (the context's material map) ----
+    def _vgrid(self, origin, dl):
+        g = _native.VoxelGrid()
+        for q in range(3):
+            g.origin[q] = float(np.float32(origin[q]))
+        g.dl = float(np.float32(dl))
+        return g
+
+    @staticmethod
+    def _f3(v):
+        return np.ascontiguousarray(np.asarray(v, np.float32).reshape(3))
+
+    def mat_fill(self, mat_id=0):
+        self._chk(self.lib.tlm_mat_fill(self.h, int(mat_id)), "tlm_mat_fill")
+
+    def mat_cells_box(self, lo, hi, mat_id):
+        a = np.ascontiguousarray(lo, np.int32).reshape(3)
+        b = np.ascontiguousarray(hi, np.int32).reshape(3)
+        self._chk(self.lib.tlm_mat_cells_box(self.h, ptr(a, C.c_int32), ptr(b, C.c_int32),
+                                             int(mat_id)), "tlm_mat_cells_box")
+
+    def voxelize_mesh(self, origin, dl, verts, idx, offset, mat_id, solid):
+        v = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+        ix = np.ascontiguousarray(idx, np.int32).reshape(-1, 3)
+        off = self._f3(offset)
+        g = self._vgrid(origin, dl)
+        self._chk(self.lib.tlm_voxelize_mesh(self.h, C.byref(g), ptr(v, C.c_float), len(v),
+                                             ptr(ix, C.c_int32), len(ix), ptr(off, C.c_float),
+                                             int(mat_id), int(bool(solid))), "tlm_voxelize_mesh")
+
+    def voxelize_wire(self, origin, dl, pts, radius, offset, mat_id):
+        p = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+        off = self._f3(offset)
+        g = self._vgrid(origin, dl)
+        self._chk(self.lib.tlm_voxelize_wire(self.h, C.byref(g), ptr(p, C.c_float), len(p),
+                                             float(np.float32(radius)), ptr(off, C.c_float),
+                                             int(mat_id)), "tlm_voxelize_wire")
+
+    def carve_feed(self, origin, dl, a, b, offset=(0, 0, 0)):
+        """Returns (sorted unique cells carved to AIR, dominant axis)."""
+        g = self._vgrid(origin, dl)
+        aa, bb, off = self._f3(a), self._f3(b), self._f3(offset)
+        pol = C.c_int32(0)
+        cap = 4096
+        cells = np.zeros(cap, np.int64)
+        n = self.lib.tlm_carve_feed(self.h, C.byref(g), ptr(aa, C.c_float), ptr(bb, C.c_float),
+                                    ptr(off, C.c_float), ptr(cells, C.c_int64), cap, C.byref(pol))
+        if n < 0:
+            self._chk(n, "tlm_carve_feed")
+        if n > cap:
+            raise TlmError("feed gap longer than 4096 cells")
+        return cells[:n].copy(), int(pol.value)
+
+    def mat_read(self) -> np.ndarray:
+        out = np.zeros(self.n, np.uint8)
+        self._chk(self.lib.tlm_mat_read(self.h, ptr(out, C.c_uint8)), "tlm_mat_read")
+        return out
+
+    def mat_write(self, mat):
+        m = np.ascontiguousarray(mat, np.uint8).reshape(-1)
+        if m.size != self.n:
+            raise TlmError("material map must hold nx*ny*nz bytes")
+        self._chk(self.lib.tlm_mat_write(self.h, ptr(m, C.c_uint8)), "tlm_mat_write")
+
+    def upload_mat(self, yhat=(), ghat=()):
+        """tlm_upload of the device material map with a zero state."""
+        yh = np.ascontiguousarray(yhat, np.float32).reshape(-1)
+        gh = np.ascontiguousarray(ghat, np.float32).reshape(-1)
+        self._chk(self.lib.tlm_upload_mat(self.h, ptr(yh, C.c_float), ptr(gh, C.c_float),
+                                          yh.size), "tlm_upload_mat")
+
+    def pec_faces_mat(self) -> int:
+        n = int(self.lib.tlm_pec_faces_mat(self.h))
+        if n < 0:
+            self._chk(n, "tlm_pec_faces_mat")
+        self.nface = n
+        return n
+
+    def huygens_mat(self, offset) -> int:
+        n = int(self.lib.tlm_huygens_mat(self.h, int(offset)))
+        if n < 0:
+            self._chk(n, "tlm_huygens_mat")
+        self.nhuy = n
+        return n
+
+    def _read_list(self, fn, name):
+        n = int(fn(self.h, None, None, 0))
+        if n < 0:
+            self._chk(n, name)
+        cells = np.zeros(n, np.int64)
+        axes = np.zeros(n, np.int32)
+        if n:
+            fn(self.h, ptr(cells, C.c_int64), ptr(axes, C.c_int32), n)
+        return cells, axes
+
+    def read_faces(self):
+        return self._read_list(self.lib.tlm_read_faces, "tlm_read_faces")
+
+    def read_huygens_faces(self):
+        return self._read_list(self.lib.tlm_read_huygens_faces, "tlm_read_huygens_faces")
+
     def set_sources(self, cells, pols, amps):
         c = np.ascontiguousarray(cells, np.int64)
         p = np.ascontiguousarray(pols, np.int32)
@@ -372,28 +473,65 @@ class SceneResult:
     total_ms: float | None = None
 
 
+def apply_ops(dev: TlmDevice, scene) -> list:
+    """Run a scene's set-up operations on the device material map (feed
+    carves last); returns the feed ports [(cells, pol, amp)]."""
+    org, dl = scene.origin, scene.dl
+    feeds = []
+    for op in scene.ops:
+        kind = op[0]
+        if kind == "fill":
+            dev.mat_fill(op[1])
+        elif kind == "cells_box":
+            dev.mat_cells_box(op[1], op[2], op[3])
+        elif kind == "mesh":
+            dev.voxelize_mesh(org, dl, op[1], op[2], op[3], op[4], op[5])
+        elif kind == "wire":
+            dev.voxelize_wire(org, dl, op[1], op[2], op[3], op[4])
+        elif kind != "feed":
+            raise ValueError(f"unknown set-up operation {kind!r}")
+    for op in scene.ops:
+        if op[0] == "feed":
+            cells, pol = dev.carve_feed(org, dl, op[1], op[2], op[3])
+            if cells.size:
+                feeds.append(([int(c) for c in cells], pol, float(op[4])))
+    return feeds
+
+
 def open_scene(scene, device=0) -> TlmDevice:
-    """Create a context and load a Scene into it (no steps run)."""
+    """Create a context and load a Scene into it (no steps run). A scene of
+    set-up operations (scene.ops) is voxelised on the device; a host map is
+    uploaded."""
     dev = TlmDevice(scene.nx, scene.ny, scene.nz, scene.dl, scene.rho, pulse=scene.pulse,
                     device=device, fp32_halo=scene.fp32_halo)
     yh, gh = stub_admittances(scene.epsr, scene.sigma, scene.dl)
-    dev.upload(scene.mat, yh, gh, scene.V0, scene.Vs0)
+    if scene.ops is not None:
+        if scene.ports or scene.V0 is not None:
+            raise ValueError("a device-built scene takes its ports from feed operations and "
+                             "starts from a zero state")
+        ports = apply_ops(dev, scene)       # carved cells are AIR by construction
+        dev.upload_mat(yh, gh)
+        mat = None
+    else:
+        dev.upload(scene.mat, yh, gh, scene.V0, scene.Vs0)
+        mat = scene.mat.reshape(-1)
+        # addPort keeps the AIR cells only
+        ports = [([int(c) for c in pc if 0 <= c < mat.size and mat[c] == 0], p, a)
+                 for pc, p, a in scene.ports]
     dt = timestep(scene.dl)
     total = scene.total_steps
     sv, cph, sph = schedule_tables(scene.waveform, scene.f0, dt, total, scene.gd_width,
                                    scene.gd_delay)
     dev.set_schedule(sv, cph, sph, scene.settle_steps)
     dev.set_energy_every(scene.energy_every)
-    # injection order: sources, then the AIR cells of each port (addPort filter)
-    mat = scene.mat.reshape(-1)
+    # injection order: sources, then the cells of each port
     cells, pols, amps = [], [], []
     for c, p, a in scene.sources:
         cells.append(c)
         pols.append(p)
         amps.append(a)
     port_list = []
-    for pc, p, a in scene.ports:
-        pc = [int(c) for c in pc if 0 <= c < mat.size and mat[c] == 0]
+    for pc, p, a in ports:
         port_list.append((pc, p))
         for c in pc:
             cells.append(c)
@@ -403,13 +541,19 @@ def open_scene(scene, device=0) -> TlmDevice:
     if port_list:
         dev.set_ports(port_list)
     if scene.faces:
-        fc, fa, _ = pec_faces(scene.nx, scene.ny, scene.nz, mat)
-        if fc.size:
-            dev.set_faces(fc, fa)
+        if mat is None:
+            dev.pec_faces_mat()
+        else:
+            fc, fa, _ = pec_faces(scene.nx, scene.ny, scene.nz, mat)
+            if fc.size:
+                dev.set_faces(fc, fa)
     if scene.huy_offset >= 2:
-        hc, ha, _ = huygens_faces(scene.nx, scene.ny, scene.nz, mat, scene.huy_offset)
-        if hc.size:
-            dev.set_huygens(hc, ha)
+        if mat is None:
+            dev.huygens_mat(scene.huy_offset)
+        else:
+            hc, ha, _ = huygens_faces(scene.nx, scene.ny, scene.nz, mat, scene.huy_offset)
+            if hc.size:
+                dev.set_huygens(hc, ha)
     if len(scene.probes):
         dev.set_probes(scene.probes, np.float32(-1.0) / np.float32(scene.dl))
         if len(scene.dft_freqs):

@@ -309,6 +309,64 @@ int tlm_device_memory(int device, size_t *free_bytes, size_t *total_bytes);
  * dielectric, cell codes and per-step work buffers; monitors excluded). */
 int64_t tlm_estimate_bytes(const tlm_grid_desc *desc, int has_stubs);
 
+/* ---------------------------------------------------------------------
+ * Scene -> voxel set-up on the device: the step's input side.
+ *
+ * A context owns a device material map (uint8 per cell, cellIndex order,
+ * MAT_AIR = 0, MAT_PEC = 1, dielectric ids >= 2), created all AIR on the
+ * first call below. The voxelisers write it in call order, with the
+ * reference's results bit for bit; the map then becomes the grid's cell
+ * codes through tlm_upload_mat, with no host pass over the cells. The
+ * reference does this on one host thread before every run
+ * (TMainForm::voxelizeScene, MainUnit.cpp:431-459: dielectrics first, then
+ * PEC, so PEC wins overlaps; the feed gap is carved after, :636-660).
+ * Refused while the context is a batch member.
+ * ------------------------------------------------------------------- */
+typedef struct tlm_voxel_grid {
+    float origin[3]; /* VoxelGridSpec::origin (Geometry.h:88): corner of cell (0,0,0) */
+    float dl;        /* VoxelGridSpec::dl; the grid extents are the context's */
+} tlm_voxel_grid;
+
+/* mat.assign(n, mat_id) (MainUnit.cpp:435). */
+int tlm_mat_fill(tlm_ctx *ctx, uint8_t mat_id);
+/* Cells lo <= (i,j,k) < hi (clipped to the grid) set to mat_id. No
+ * counterpart in the reference: the synthetic dielectric-box and dipole
+ * scenes are cell ranges. */
+int tlm_mat_cells_box(tlm_ctx *ctx, const int32_t lo[3], const int32_t hi[3], uint8_t mat_id);
+/* VoxelizeMesh (Geometry.cpp:369-397; solid = 0) or VoxelizeMeshSolid
+ * (Geometry.cpp:551-606; solid = 1) of the triangle mesh (verts: 3 floats
+ * per vertex, idx: 3 vertex indices per triangle) translated by offset. */
+int tlm_voxelize_mesh(tlm_ctx *ctx, const tlm_voxel_grid *g, const float *verts, int32_t nverts,
+                      const int32_t *idx, int32_t ntri, const float offset[3], uint8_t mat_id,
+                      int solid);
+/* VoxelizeWire (Geometry.cpp:608-642): polyline pts (3 floats each) of the
+ * given radius, translated by offset. */
+int tlm_voxelize_wire(tlm_ctx *ctx, const tlm_voxel_grid *g, const float *pts, int32_t npts,
+                      float radius, const float offset[3], uint8_t mat_id);
+/* Feed gap of a fed object (MainUnit.cpp:636-660): the cells sampled along
+ * the inner 70% of (a+offset)->(b+offset) are carved to AIR; returns their
+ * count (sorted, unique, at most cap written to cells) and the dominant axis
+ * of b-a in *pol, or a negative TLM_ERR_*. */
+int tlm_carve_feed(tlm_ctx *ctx, const tlm_voxel_grid *g, const float a[3], const float b[3],
+                   const float offset[3], int64_t *cells, int32_t cap, int32_t *pol);
+/* Copies between the device map and a host array of nx*ny*nz bytes. */
+int tlm_mat_read(tlm_ctx *ctx, uint8_t *mat);
+int tlm_mat_write(tlm_ctx *ctx, const uint8_t *mat);
+/* tlm_upload with the device map as mat and a zero state (TlmSolver::setup,
+ * TlmSolver.cpp:259-304): cell codes, air-block flags and the dielectric
+ * check are computed on the device. */
+int tlm_upload_mat(tlm_ctx *ctx, const float *yhat, const float *ghat, int nmat);
+/* BuildPecSurfaceFaces (Geometry.cpp:402-436) on the device map, installed
+ * as the surface-current faces (as tlm_set_faces). Returns the face count
+ * or a negative TLM_ERR_*. */
+int64_t tlm_pec_faces_mat(tlm_ctx *ctx);
+/* TlmSolver::buildHuygens (TlmSolver.cpp:313-359) on the device map,
+ * installed as the Huygens faces (as tlm_set_huygens). Returns the count. */
+int64_t tlm_huygens_mat(tlm_ctx *ctx, int offset);
+/* Installed face lists (unpadded cells, axes); return the count. */
+int64_t tlm_read_faces(tlm_ctx *ctx, int64_t *cells, int32_t *axes, int64_t cap);
+int64_t tlm_read_huygens_faces(tlm_ctx *ctx, int64_t *cells, int32_t *axes, int64_t cap);
+
 #ifdef __cplusplus
 }
 #endif

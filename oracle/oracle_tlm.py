@@ -124,6 +124,15 @@ def load_library():
         "ot_encode": (None, [C.c_int, fp, C.c_float, C.POINTER(C.c_uint16), C.c_int64]),
         "ot_scatter_unitarity_error": (C.c_float, []),
         "ot_diel_scatter_unitarity_error": (C.c_float, [C.c_float]),
+        "ot_voxelize_mesh": (None, [C.c_int, C.c_int, C.c_int, fp, C.c_float, fp, C.c_int, i32p,
+                                    C.c_int, fp, C.c_uint8, C.c_int, C.POINTER(C.c_uint8)]),
+        "ot_voxelize_wire": (None, [C.c_int, C.c_int, C.c_int, fp, C.c_float, fp, C.c_int,
+                                    C.c_float, fp, C.c_uint8, C.POINTER(C.c_uint8)]),
+        "ot_tri_box_overlap": (C.c_int, [fp, fp, fp]),
+        "ot_ray_x_tri": (C.c_int, [fp, fp, fp]),
+        "ot_box_mesh": (None, [fp, fp, fp, i32p]),
+        "ot_carve_feed": (C.c_int, [C.c_int, C.c_int, C.c_int, fp, C.c_float, fp, fp, fp,
+                                    C.POINTER(C.c_uint8), i64p, C.c_int, i32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -394,3 +403,75 @@ def encode_pulses(fmt: int, values, inv_scale: float = 1.0) -> np.ndarray:
 
 def plan_exponent(bound: float) -> int:
     return int(load_library().ot_plan_exponent(C.c_float(bound)))
+
+
+# --- voxel set-up (Geometry.cpp:306-642) -----------------------------------
+
+def _f3(v):
+    return np.ascontiguousarray(np.asarray(v, np.float32).reshape(3))
+
+
+def voxelize_mesh(mat, dims, origin, dl, verts, idx, offset, mat_id, solid):
+    """VoxelizeMesh / VoxelizeMeshSolid into `mat` (uint8, cellIndex order,
+    updated in place)."""
+    nx, ny, nz = dims
+    assert mat.dtype == np.uint8 and mat.size == nx * ny * nz and mat.flags.c_contiguous
+    v = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+    ix = np.ascontiguousarray(idx, np.int32).reshape(-1, 3)
+    o, off = _f3(origin), _f3(offset)
+    load_library().ot_voxelize_mesh(nx, ny, nz, _ptr(o, C.c_float), C.c_float(dl),
+                                    _ptr(v, C.c_float), len(v), _ptr(ix, C.c_int32), len(ix),
+                                    _ptr(off, C.c_float), int(mat_id), int(bool(solid)),
+                                    _ptr(mat, C.c_uint8))
+    return mat
+
+
+def voxelize_wire(mat, dims, origin, dl, pts, radius, offset, mat_id):
+    """VoxelizeWire into `mat` (updated in place)."""
+    nx, ny, nz = dims
+    assert mat.dtype == np.uint8 and mat.size == nx * ny * nz and mat.flags.c_contiguous
+    p = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+    o, off = _f3(origin), _f3(offset)
+    load_library().ot_voxelize_wire(nx, ny, nz, _ptr(o, C.c_float), C.c_float(dl),
+                                    _ptr(p, C.c_float), len(p), C.c_float(radius),
+                                    _ptr(off, C.c_float), int(mat_id), _ptr(mat, C.c_uint8))
+    return mat
+
+
+def tri_box_overlap(center, half, tri) -> bool:
+    c, h = _f3(center), _f3(half)
+    t = np.ascontiguousarray(tri, np.float32).reshape(9)
+    return bool(load_library().ot_tri_box_overlap(_ptr(c, C.c_float), _ptr(h, C.c_float),
+                                                  _ptr(t, C.c_float)))
+
+
+def ray_x_tri(o, tri):
+    """+x ray from o against a triangle: the hit x or None."""
+    oo = _f3(o)
+    t = np.ascontiguousarray(tri, np.float32).reshape(9)
+    hx = C.c_float(0)
+    ok = load_library().ot_ray_x_tri(_ptr(oo, C.c_float), _ptr(t, C.c_float), C.byref(hx))
+    return float(hx.value) if ok else None
+
+
+def box_mesh(lo, hi):
+    """TriMesh::addBox as 8 shared vertices and 12 triangles."""
+    v = np.zeros((8, 3), np.float32)
+    ix = np.zeros((12, 3), np.int32)
+    load_library().ot_box_mesh(_ptr(_f3(lo), C.c_float), _ptr(_f3(hi), C.c_float),
+                               _ptr(v, C.c_float), _ptr(ix, C.c_int32))
+    return v, ix
+
+
+def carve_feed(mat, dims, origin, dl, a, b, offset=(0, 0, 0)):
+    """Feed-gap carve (MainUnit.cpp:636-660) into `mat`; returns (cells, pol)."""
+    nx, ny, nz = dims
+    cap = 4096
+    cells = np.zeros(cap, np.int64)
+    pol = C.c_int32(0)
+    o, aa, bb, off = _f3(origin), _f3(a), _f3(b), _f3(offset)
+    n = load_library().ot_carve_feed(nx, ny, nz, _ptr(o, C.c_float), C.c_float(dl),
+                                     _ptr(aa, C.c_float), _ptr(bb, C.c_float),
+                                     _ptr(off, C.c_float), _ptr(mat, C.c_uint8),
+                                     _ptr(cells, C.c_int64), cap, C.byref(pol))
+    return cells[:min(n, cap)].copy(), int(pol.value)

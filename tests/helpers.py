@@ -138,3 +138,31 @@ def compare_results(gpu: SceneResult, ref: SceneResult, scene, state=True, energ
         # the float history the reference keeps
         assert np.array_equal(gpu.energy.astype(np.float32), ref.energy.astype(np.float32)) or \
             rel.max() <= 1e-15
+
+
+def oracle_build(scene):
+    """The host-map scene equal to a device-built one: its set-up operations
+    run by the oracle's voxelisers and feed carve (cell boxes and fills are
+    plain array stores), the feeds as ports."""
+    import dataclasses
+
+    from emsolver_amd import scenes as _scenes
+    from oracle import oracle_tlm as ot
+    dims = (scene.nx, scene.ny, scene.nz)
+    mat = np.zeros(scene.ncell, np.uint8)
+    for op in scene.ops:
+        if op[0] == "fill":
+            mat[:] = op[1]
+        elif op[0] == "cells_box":
+            _scenes.stamp_box(mat, *dims, op[1], op[2], op[3])
+        elif op[0] == "mesh":
+            ot.voxelize_mesh(mat, dims, scene.origin, scene.dl, op[1], op[2], op[3], op[4], op[5])
+        elif op[0] == "wire":
+            ot.voxelize_wire(mat, dims, scene.origin, scene.dl, op[1], op[2], op[3], op[4])
+    ports = []
+    for op in scene.ops:
+        if op[0] == "feed":
+            cells, pol = ot.carve_feed(mat, dims, scene.origin, scene.dl, op[1], op[2], op[3])
+            if cells.size:
+                ports.append(([int(c) for c in cells], pol, float(op[4])))
+    return dataclasses.replace(scene, mat=mat, ops=None, ports=ports)

@@ -110,3 +110,22 @@ def test_stub_admittances_float_rule():
     assert yh[0] == 0 and yh[1] == 0 and yh[2] == np.float32(8.0)
     assert gh[0] == 0 and gh[1] == 0
     assert abs(gh[3] - 0.05 * scenes.DL * 376.730313) < 1e-6
+
+
+def test_voxel_setup_rejects_null_context():
+    lib = _native.load()
+    g = _native.VoxelGrid()
+    g.dl = 0.01
+    z = np.zeros(3, np.float32)
+    zp = z.ctypes.data_as(C.POINTER(C.c_float))
+    assert lib.tlm_mat_fill(None, 0) == -1
+    assert lib.tlm_mat_cells_box(None, None, None, 1) == -1
+    assert lib.tlm_voxelize_mesh(None, C.byref(g), zp, 1, None, 0, zp, 2, 1) == -1
+    assert lib.tlm_voxelize_wire(None, C.byref(g), zp, 1, 0.0, zp, 1) == -1
+    assert lib.tlm_carve_feed(None, C.byref(g), zp, zp, zp, None, 0, None) == -1
+    assert lib.tlm_mat_read(None, None) == -1
+    assert lib.tlm_upload_mat(None, None, None, 0) == -1
+    assert lib.tlm_pec_faces_mat(None) == -1
+    assert lib.tlm_huygens_mat(None, 4) == -1
+    assert lib.tlm_read_faces(None, None, None, 0) == -1
+    assert "null context" in lib.tlm_last_error(None).decode()

@@ -50,3 +50,24 @@ def test_admission_waits_for_memory():
     done = srv.serve(reqs, realtime=False)
     assert len(done) == 4
     assert srv.stats["mean_members"] <= 1.0 + 1e-9
+
+
+def test_served_device_setup_equals_host_setup():
+    """Requests voxelised on the device at admission give the bits of the
+    same requests uploaded as host maps."""
+    def stream(dev_setup):
+        reqs = serve.served_stream(count=6, rate=1e9, lo=12, hi=36, steps=(30, 90), seed=33,
+                                   device_setup=dev_setup)
+        for r in reqs:
+            r.arrival = 0.0
+        return reqs
+    out = {}
+    for dev_setup in (False, True):
+        srv = serve.TlmServer(block_ticks=16, max_members=3, hbm_reserve=0)
+        out[dev_setup] = {c.rid: c for c in srv.serve(stream(dev_setup), realtime=False)}
+    assert sorted(out[True]) == list(range(6))
+    for rid, c in out[True].items():
+        h = out[False][rid]
+        assert np.abs(h.ports).max() > 0
+        helpers.assert_bitexact(f"req{rid} probes", c.probes, h.probes)
+        helpers.assert_bitexact(f"req{rid} ports", c.ports, h.ports)
