@@ -291,6 +291,58 @@ def precision_report(steps=4000, crop=128):
     return rep
 
 
+def setup_run(n=256, subdiv=5):
+    """Scene -> voxel set-up of mesh_scene(n) (dielectric sphere of
+    20*4^subdiv triangles, as an imported CAD mesh would be, box and rotated
+    box, PEC plate, thick bent wire, fed thin dipole): on the device
+    (voxelisers, cell codes, PEC faces, Huygens faces) against the host path
+    a reference user takes (the voxelisers on one CPU thread, the oracle's
+    restatement of Geometry.cpp, then the host upload and host face
+    builds)."""
+    from emsolver_amd import scenes, tlm
+    from tests import helpers
+    sc = scenes.mesh_scene(n, steps=4, subdiv=subdiv)
+    ntri = sum(len(op[2]) for op in sc.ops if op[0] == "mesh")
+    yh, gh = tlm.stub_admittances(sc.epsr, sc.sigma, sc.dl)
+    best = None
+    for _ in range(3):
+        with tlm.TlmDevice(sc.nx, sc.ny, sc.nz, sc.dl) as dev:
+            dev.sync()
+            t0 = time.perf_counter()
+            tlm.apply_ops(dev, sc)
+            dev.upload_mat(yh, gh)
+            nf = dev.pec_faces_mat()
+            nh = dev.huygens_mat(sc.huy_offset)
+            dev.sync()
+            dt = time.perf_counter() - t0
+            dmat = dev.mat_read()
+        best = dt if best is None else min(best, dt)
+    t0 = time.perf_counter()
+    host = helpers.oracle_build(sc)           # the voxelisers on one CPU thread
+    t_vox = time.perf_counter() - t0
+    assert np.array_equal(host.mat, dmat), "device map differs from the CPU voxelisers"
+    with tlm.TlmDevice(sc.nx, sc.ny, sc.nz, sc.dl) as dev:
+        dev.sync()
+        t0 = time.perf_counter()
+        dev.upload(host.mat, yh, gh)
+        fc, fa, _ = tlm.pec_faces(sc.nx, sc.ny, sc.nz, host.mat)
+        dev.set_faces(fc, fa)
+        hc, ha, _ = tlm.huygens_faces(sc.nx, sc.ny, sc.nz, host.mat, sc.huy_offset)
+        dev.set_huygens(hc, ha)
+        dev.sync()
+        t_up = time.perf_counter() - t0
+    assert fc.size == nf and hc.size == nh
+    out = {"grid": [n, n, n], "triangles": ntri, "faces": int(nf), "huygens_faces": int(nh),
+           "device_setup_ms": round(best * 1e3, 2),
+           "cpu_voxelize_ms": round(t_vox * 1e3, 1),
+           "host_upload_and_faces_ms": round(t_up * 1e3, 1),
+           "speedup": round((t_vox + t_up) / best, 1),
+           "cpu_sample": "oracle voxelisers, 1 thread (TMainForm::voxelizeScene is serial)"}
+    log(f"[bench] scene_setup_{n}: device {out['device_setup_ms']} ms vs CPU voxelise "
+        f"{out['cpu_voxelize_ms']} ms + host upload/faces {out['host_upload_and_faces_ms']} ms")
+    return out
+
+
 def cpu_baseline(scene, budget_s=15.0):
     from tests import helpers
     o = helpers.oracle_open(scene.with_steps(10**6))
@@ -329,7 +381,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched and served lines")
     ap.add_argument("--lines", default="",
                     help="comma list of extra lines to run (diel_boxes_512_f32, diel_boxes_512_f16, "
-                         "diel_boxes_512_bf16, free_space_512_f32, batched_256, served); default all")
+                         "diel_boxes_512_bf16, free_space_512_f32, scene_setup_256, "
+                         "scene_setup_512, batched_256, served); default all")
     ap.add_argument("--no-graph", action="store_true",
                     help="plain launches instead of captured HIP graphs in the batched lines "
                          "(for tracing tools that do not follow graph launches)")
@@ -417,6 +470,9 @@ def main():
         log(f"[bench] free_space_512: {r['gcells']:.2f} Gcell/s, kernel {r['kernel_ms_avg']:.3f} ms, "
             f"{r['achieved_gbs']:.0f} GB/s = {r['achieved_gbs'] / HBM_PEAK_GBS:.3f} of peak")
 
+    for sn in (256, 512):
+        if not args.no_extra and want(f"scene_setup_{sn}"):
+            extra[f"scene_setup_{sn}"] = setup_run(sn)
     if not args.no_batch and want("batched_256"):
         extra["batched_256"] = batched_run(256)
     if not args.no_batch and want("served"):

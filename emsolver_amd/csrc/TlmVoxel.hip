@@ -436,8 +436,109 @@ struct Rows
     float xs;
 };
 
-// Hit count (HITS = false) or hits written, sorted and filled (HITS = true)
-// for one row per thread; the triangles pass through LDS 256 at a time.
+// Sorts a row's hits (ascending, as std::sort) and fills the cells whose
+// centre lies between the hits of each pair (Geometry.cpp:586-604).
+__device__ void fill_row(float *h, long long nh, long long row, const Rows &rw, const VGrid &g,
+                         uint8_t *mat, uint8_t id)
+{
+    if (nh < 2)
+        return;
+    for (long long a = 1; a < nh; ++a) // insertion sort
+    {
+        const float v = h[a];
+        long long b = a - 1;
+        while (b >= 0 && h[b] > v)
+        {
+            h[b + 1] = h[b];
+            --b;
+        }
+        h[b + 1] = v;
+    }
+    const int j = rw.j0 + (int)(row % rw.nj);
+    const int k = rw.k0 + (int)(row / rw.nj);
+    for (long long p = 0; p + 1 < nh; p += 2)
+    {
+        const float xa = h[p], xb = h[p + 1];
+        int ia, ib, d1, d2;
+        g.cell_of(mk(xa, 0.f, 0.f), ia, d1, d2);
+        g.cell_of(mk(xb, 0.f, 0.f), ib, d1, d2);
+        ia = max(ia, rw.i0);
+        ib = min(ib, rw.i1);
+        for (int i = ia; i <= ib; ++i)
+        {
+            const float cx = g.o.x + (i + 0.5f) * g.dl;
+            if (cx >= xa && cx <= xb)
+                mat[g.index(i, j, k)] = id;
+        }
+    }
+}
+
+__device__ inline F3 row_origin(long long row, const Rows &rw, const VGrid &g)
+{
+    const int j = rw.j0 + (int)(row % rw.nj);
+    const int k = rw.k0 + (int)(row / rw.nj);
+    F3 o = g.center(rw.i0, j, k);
+    o.x = rw.xs;
+    return o;
+}
+
+constexpr int kRowCap = 16; // hit slots per row of the one-pass fill
+
+// One pass over (block of 256 rows) x (chunk of triangles): each hit takes
+// the next slot of its row (the order is restored by the sort); hits past
+// kRowCap are counted but not kept.
+__global__ void __launch_bounds__(kThreads)
+    k_ray_slots(const float *tri, int ntri, int per_chunk, Rows rw, VGrid g, int *cnt,
+                float *slots)
+{
+    __shared__ float st[kThreads * 9];
+    const long long row = (long long)blockIdx.x * kThreads + threadIdx.x;
+    const bool live = row < rw.n;
+    const F3 o = live ? row_origin(row, rw, g) : mk(0.f, 0.f, 0.f);
+    const int t_beg = blockIdx.y * per_chunk;
+    const int t_end = min(ntri, t_beg + per_chunk);
+    for (int t0 = t_beg; t0 < t_end; t0 += kThreads)
+    {
+        const int m = min(kThreads, t_end - t0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < m * 9; e += kThreads)
+            st[e] = tri[9 * (long long)t0 + e];
+        __syncthreads();
+        if (!live)
+            continue;
+        for (int t = 0; t < m; ++t)
+        {
+            float hx;
+            if (ray_x_tri(o, ld3(st + 9 * t), ld3(st + 9 * t + 3), ld3(st + 9 * t + 6), hx))
+            {
+                const int slot = atomicAdd(&cnt[row], 1);
+                if (slot < kRowCap)
+                    slots[row * kRowCap + slot] = hx;
+            }
+        }
+    }
+}
+
+// Fill of the rows whose hits all fit their slots; *overflow = 1 if a row's
+// did not (those rows are left to the two-pass fill).
+__global__ void k_ray_fill(const int *cnt, float *slots, Rows rw, VGrid g, uint8_t *mat,
+                           uint8_t id, int *overflow)
+{
+    const long long row = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= rw.n)
+        return;
+    const int n = cnt[row];
+    if (n > kRowCap)
+    {
+        *overflow = 1;
+        return;
+    }
+    fill_row(slots + row * kRowCap, n, row, rw, g, mat, id);
+}
+
+// Two-pass fill for any hit count: counts (HITS = false), then the hits
+// written at the row's offset, sorted and filled (HITS = true); one row per
+// thread, the triangles pass through LDS 256 at a time.
 template <bool HITS>
 __global__ void __launch_bounds__(kThreads)
     k_rows(const float *tri, int ntri, Rows rw, VGrid g, long long *cnt, const long long *off,
@@ -446,14 +547,7 @@ __global__ void __launch_bounds__(kThreads)
     __shared__ float st[kThreads * 9];
     const long long row = (long long)blockIdx.x * kThreads + threadIdx.x;
     const bool live = row < rw.n;
-    F3 o = mk(0.f, 0.f, 0.f);
-    if (live)
-    {
-        const int j = rw.j0 + (int)(row % rw.nj);
-        const int k = rw.k0 + (int)(row / rw.nj);
-        o = g.center(rw.i0, j, k);
-        o.x = rw.xs;
-    }
+    const F3 o = live ? row_origin(row, rw, g) : mk(0.f, 0.f, 0.f);
     long long nh = 0;
     float *h = HITS && live ? hits + off[row] : nullptr;
     // the count pass gave this row's room; never write past it
@@ -485,38 +579,7 @@ __global__ void __launch_bounds__(kThreads)
         cnt[row] = nh;
         return;
     }
-    nh = min(nh, room);
-    if (nh < 2)
-        return;
-    // insertion sort of the row's hits (ascending, as std::sort)
-    for (long long a = 1; a < nh; ++a)
-    {
-        const float v = h[a];
-        long long b = a - 1;
-        while (b >= 0 && h[b] > v)
-        {
-            h[b + 1] = h[b];
-            --b;
-        }
-        h[b + 1] = v;
-    }
-    const int j = rw.j0 + (int)(row % rw.nj);
-    const int k = rw.k0 + (int)(row / rw.nj);
-    for (long long p = 0; p + 1 < nh; p += 2)
-    {
-        const float xa = h[p], xb = h[p + 1];
-        int ia, ib, d1, d2;
-        g.cell_of(mk(xa, 0.f, 0.f), ia, d1, d2);
-        g.cell_of(mk(xb, 0.f, 0.f), ib, d1, d2);
-        ia = max(ia, rw.i0);
-        ib = min(ib, rw.i1);
-        for (int i = ia; i <= ib; ++i)
-        {
-            const float cx = g.o.x + (i + 0.5f) * g.dl;
-            if (cx >= xa && cx <= xb)
-                mat[g.index(i, j, k)] = id;
-        }
-    }
+    fill_row(h, min(nh, room), row, rw, g, mat, id);
 }
 
 // One thread per (sample, cell of the (2rc+1)^3 stamp cube) of a wire
@@ -562,36 +625,38 @@ __global__ void k_set_cells(uint8_t *mat, const long long *cells, int n, uint8_t
 
 // Cell codes of the padded layout: material id | neighbour-PEC bits; the
 // slab padding is coded PEC (the host build in tlm_upload, on the device).
+// Block r < ny*nz codes row (j, k) = (r % ny, r / ny); block ny*nz + k the
+// padding of slab k.
 __global__ void k_codes(const uint8_t *mat, int nx, int ny, int nz, long long nxy, long long Pp,
-                        long long P, uint16_t *code)
+                        uint16_t *code)
 {
-    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < P;
-         p += (long long)gridDim.x * blockDim.x)
+    const int r = blockIdx.x;
+    if (r >= ny * nz)
     {
-        const long long k = p / Pp;
-        const long long r = p - k * Pp;
-        if (r >= nxy)
-        {
-            code[p] = 1;
-            continue;
-        }
-        const int j = (int)(r / nx);
-        const int i = (int)(r - (long long)j * nx);
-        const uint8_t *row = mat + k * nxy + r;
-        unsigned v = row[0];
-        if (i > 0 && row[-1] == 1)
+        const long long base = (long long)(r - ny * nz) * Pp;
+        for (long long p = nxy + threadIdx.x; p < Pp; p += blockDim.x)
+            code[base + p] = 1;
+        return;
+    }
+    const int k = r / ny, j = r - k * ny;
+    const uint8_t *row = mat + (long long)k * nxy + (long long)j * nx;
+    uint16_t *out = code + (long long)k * Pp + (long long)j * nx;
+    for (int i = threadIdx.x; i < nx; i += blockDim.x)
+    {
+        unsigned v = row[i];
+        if (i > 0 && row[i - 1] == 1)
             v |= tlm::NB_XM;
-        if (i < nx - 1 && row[1] == 1)
+        if (i < nx - 1 && row[i + 1] == 1)
             v |= tlm::NB_XP;
-        if (j > 0 && row[-nx] == 1)
+        if (j > 0 && row[i - nx] == 1)
             v |= tlm::NB_YM;
-        if (j < ny - 1 && row[nx] == 1)
+        if (j < ny - 1 && row[i + nx] == 1)
             v |= tlm::NB_YP;
-        if (k > 0 && row[-nxy] == 1)
+        if (k > 0 && row[i - nxy] == 1)
             v |= tlm::NB_ZM;
-        if (k < nz - 1 && row[nxy] == 1)
+        if (k < nz - 1 && row[i + nxy] == 1)
             v |= tlm::NB_ZP;
-        code[p] = (uint16_t)v;
+        out[i] = (uint16_t)v;
     }
 }
 
@@ -611,53 +676,159 @@ __global__ void k_uair(const uint16_t *code, long long nblk, uint8_t *uair)
     uair[b] = any == 0;
 }
 
-// Largest dielectric index (id - 2) of the map, -1 if none.
+// Largest dielectric index (id - 2) of the map, -1 if none (16 cells per
+// load; the map is a 256-byte aligned allocation).
 __global__ void k_max_diel(const uint8_t *mat, long long n, int *out)
 {
-    int m = -1;
-    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+    unsigned m = 0; // largest id seen
+    const long long n16 = n / 16;
+    const uint4 *v = reinterpret_cast<const uint4 *>(mat);
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n16;
          q += (long long)gridDim.x * blockDim.x)
-        if (mat[q] >= 2)
-            m = max(m, (int)mat[q] - 2);
-    m = __reduce_max_sync(~0ull, m);
-    if ((threadIdx.x & (tlm::kWave - 1)) == 0 && m >= 0)
-        atomicMax(out, m);
+    {
+        const uint4 w = v[q];
+        const unsigned x[4] = {w.x, w.y, w.z, w.w};
+        for (int e = 0; e < 4; ++e)
+            for (int b = 0; b < 32; b += 8)
+                m = max(m, (x[e] >> b) & 0xffu);
+    }
+    if (blockIdx.x == 0)
+        for (long long q = n16 * 16 + threadIdx.x; q < n; q += blockDim.x)
+            m = max(m, (unsigned)mat[q]);
+    int d = m >= 2 ? (int)m - 2 : -1;
+    d = __reduce_max_sync(~0ull, d);
+    if ((threadIdx.x & (tlm::kWave - 1)) == 0 && d >= 0)
+        atomicMax(out, d);
 }
 
-// ---- ordered compactions: PEC surface faces and Huygens faces ----
+// ---- ordered face lists: PEC surface faces and Huygens faces ----
 
-// Candidate q = cell*6 + axis*2 + (sign > 0), in cellIndex order, as the
-// loops of BuildPecSurfaceFaces (Geometry.cpp:407-433).
-struct PecFaces
+// PEC faces of a PEC cell (i,j,k): bit axis*2 + (sign > 0) set where the
+// neighbour across that face exists and is not PEC, the face order of
+// BuildPecSurfaceFaces (Geometry.cpp:407-433).
+__device__ inline unsigned pec_face_bits(const uint8_t *mat, long long c, int i, int j, int k,
+                                         int nx, int ny, int nz, long long nxy)
 {
-    const uint8_t *mat;
-    int nx, ny, nz;
-    long long nxy;
-    __device__ bool operator()(long long q, long long &cell, int &axis) const
+    unsigned b = 0;
+    if (i > 0 && mat[c - 1] != 1)
+        b |= 1u;
+    if (i < nx - 1 && mat[c + 1] != 1)
+        b |= 2u;
+    if (j > 0 && mat[c - nx] != 1)
+        b |= 4u;
+    if (j < ny - 1 && mat[c + nx] != 1)
+        b |= 8u;
+    if (k > 0 && mat[c - nxy] != 1)
+        b |= 16u;
+    if (k < nz - 1 && mat[c + nxy] != 1)
+        b |= 32u;
+    return b;
+}
+
+// PEC surface faces in cellIndex order: per thread kPer consecutive cells
+// (one 16-byte load of the map). EMIT = false: face count per tile;
+// EMIT = true: the faces (padded air cell, axis) at the tile's offset.
+template <bool EMIT>
+__global__ void __launch_bounds__(kThreads)
+    k_pec_faces(const uint8_t *mat, int nx, int ny, int nz, long long nxy, long long n,
+                long long *tcnt, const long long *toff, long long Pp, long long *cells, int *axes)
+{
+    static_assert(kPer == 16, "one uint4 of the map per thread");
+    __shared__ long long lds[kThreads];
+    const long long b = (long long)blockIdx.x * kTile + (long long)threadIdx.x * kPer;
+    uint8_t m[kPer];
+    if (b + kPer <= n)
     {
-        const long long c = q / 6;
-        const int r = (int)(q - c * 6);
-        if (mat[c] != 1)
-            return false;
-        const int a = r >> 1;
-        const int s = (r & 1) ? 1 : -1;
-        const long long k = c / nxy;
-        const long long rr = c - k * nxy;
-        const int j = (int)(rr / nx);
-        const int i = (int)(rr - (long long)j * nx);
-        const int co = a == 0 ? i : (a == 1 ? j : (int)k);
-        const int nn = a == 0 ? nx : (a == 1 ? ny : nz);
-        if (co + s < 0 || co + s >= nn)
-            return false;
-        const long long step = a == 0 ? 1 : (a == 1 ? (long long)nx : nxy);
-        const long long nb = c + s * step;
-        if (mat[nb] == 1)
-            return false;
-        cell = nb;
-        axis = a;
-        return true;
+        const uint4 w = *reinterpret_cast<const uint4 *>(mat + b);
+        const unsigned x[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int r = 0; r < kPer; ++r)
+            m[r] = (uint8_t)(x[r >> 2] >> (8 * (r & 3)));
     }
-};
+    else
+    {
+#pragma unroll
+        for (int r = 0; r < kPer; ++r)
+            m[r] = b + r < n ? mat[b + r] : 0;
+    }
+    bool any = false;
+#pragma unroll
+    for (int r = 0; r < kPer; ++r)
+        any |= m[r] == 1;
+    unsigned bits[kPer];
+    long long cnt = 0;
+    int i0 = 0, j0 = 0, k0 = 0;
+    if (any)
+    {
+        k0 = (int)(b / nxy);
+        const long long rem = b - (long long)k0 * nxy;
+        j0 = (int)(rem / nx);
+        i0 = (int)(rem - (long long)j0 * nx);
+    }
+    {
+        int i = i0, j = j0, k = k0;
+#pragma unroll
+        for (int r = 0; r < kPer; ++r)
+        {
+            bits[r] = 0;
+            if (m[r] == 1)
+            {
+                bits[r] = pec_face_bits(mat, b + r, i, j, k, nx, ny, nz, nxy);
+                cnt += __builtin_popcount(bits[r]);
+            }
+            if (++i == nx)
+            {
+                i = 0;
+                if (++j == ny)
+                {
+                    j = 0;
+                    ++k;
+                }
+            }
+        }
+    }
+    long long tot;
+    long long w = block_exclusive(cnt, lds, tot);
+    if (!EMIT)
+    {
+        if (threadIdx.x == 0)
+            tcnt[blockIdx.x] = tot;
+        return;
+    }
+    if (!any)
+        return;
+    w += toff[blockIdx.x];
+    int i = i0, j = j0, k = k0;
+#pragma unroll
+    for (int r = 0; r < kPer; ++r)
+    {
+        if (bits[r])
+        {
+            const long long rowoff = (long long)j * nx + i;
+            for (int f = 0; f < 6; ++f)
+            {
+                if (!((bits[r] >> f) & 1u))
+                    continue;
+                const int a = f >> 1;
+                const int sg = (f & 1) ? 1 : -1;
+                const long long kk = k + (a == 2 ? sg : 0);
+                const long long ro = rowoff + (a == 0 ? sg : (a == 1 ? sg * (long long)nx : 0));
+                cells[w] = kk * Pp + ro; // padded device address of the air cell
+                axes[w] = a;
+                ++w;
+            }
+        }
+        if (++i == nx)
+        {
+            i = 0;
+            if (++j == ny)
+            {
+                j = 0;
+                ++k;
+            }
+        }
+    }
+}
 
 // The three face lists of TlmSolver::buildHuygens (TlmSolver.cpp:337-358),
 // concatenated: x faces per (k,j), y faces per (k,i), z faces per (j,i),
@@ -955,10 +1126,37 @@ int tlm_voxelize_mesh(tlm_ctx *c, const tlm_voxel_grid *gd, const float *verts, 
     rw.n = (long long)rw.nj * (k1 - k0 + 1);
     rw.xs = mb.lo.x - g.dl;
     rw.nh = 0;
+    const unsigned nb = (unsigned)((rw.n + kThreads - 1) / kThreads);
+    {
+        // one pass with kRowCap hit slots per row; enough triangle chunks
+        // that the grid fills the chip
+        int *scnt, *oflow;
+        float *slots;
+        if ((rc = s.get(c, &scnt, rw.n)) || (rc = s.get(c, &oflow, 1)) ||
+            (rc = s.get(c, &slots, (size_t)rw.n * kRowCap)))
+            return rc;
+        HIPCHK(c, hipMemsetAsync(scnt, 0, (size_t)rw.n * sizeof(int), c->stream));
+        HIPCHK(c, hipMemsetAsync(oflow, 0, sizeof(int), c->stream));
+        const int max_chunks = (ntri + kThreads - 1) / kThreads;
+        const int nchunk = (int)std::max(1LL, std::min<long long>(max_chunks,
+                                                                  (4096 + nb - 1) / nb));
+        const int per_chunk = (ntri + nchunk - 1) / nchunk;
+        k_ray_slots<<<dim3(nb, (unsigned)nchunk), kThreads, 0, c->stream>>>(tri, ntri, per_chunk,
+                                                                           rw, g, scnt, slots);
+        HIPCHK(c, hipGetLastError());
+        k_ray_fill<<<nb, kThreads, 0, c->stream>>>(scnt, slots, rw, g, c->mat, id, oflow);
+        HIPCHK(c, hipGetLastError());
+        int over = 0;
+        HIPCHK(c, hipMemcpyAsync(&over, oflow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (!over)
+            return TLM_OK;
+    }
+    // some row crossed the surface more than kRowCap times: the exact
+    // two-pass fill (rows already filled are filled again with the same id)
     long long *rcnt, *roff, *htot;
     if ((rc = s.get(c, &rcnt, rw.n)) || (rc = s.get(c, &roff, rw.n)) || (rc = s.get(c, &htot, 1)))
         return rc;
-    const unsigned nb = (unsigned)((rw.n + kThreads - 1) / kThreads);
     k_rows<false><<<nb, kThreads, 0, c->stream>>>(tri, ntri, rw, g, rcnt, nullptr, nullptr,
                                                   nullptr, id);
     HIPCHK(c, hipGetLastError());
@@ -1144,8 +1342,8 @@ int tlm_upload_mat(tlm_ctx *c, const float *yhat, const float *ghat, int nmat)
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (max_id >= nmat)
         return fail(c, TLM_ERR_ARG, "material id without a yhat/ghat entry");
-    k_codes<<<grid_for(c->P), kThreads, 0, c->stream>>>(c->mat, c->nx, c->ny, c->nz, c->nxy, c->Pp,
-                                                         c->P, c->code);
+    k_codes<<<(unsigned)(c->ny * c->nz + c->nz), 128, 0, c->stream>>>(c->mat, c->nx, c->ny, c->nz,
+                                                                       c->nxy, c->Pp, c->code);
     HIPCHK(c, hipGetLastError());
     const long long nblk = c->P / tlm::kWave;
     if ((rc = upload(c, &c->uair, (const uint8_t *)nullptr, (size_t)nblk)))
@@ -1162,20 +1360,33 @@ int64_t tlm_pec_faces_mat(tlm_ctx *c)
     int rc = ready(c, "tlm_pec_faces_mat");
     if (rc)
         return rc;
-    PecFaces pred;
-    pred.mat = c->mat;
-    pred.nx = c->nx;
-    pred.ny = c->ny;
-    pred.nz = c->nz;
-    pred.nxy = c->nxy;
     long long *fc = nullptr;
     int *fa = nullptr;
     long long n = 0;
-    if ((rc = compact(c, pred, 6LL * c->nx * c->ny * c->nz, &fc, &fa, &n)))
     {
-        dfree(c, fc);
-        dfree(c, fa);
-        return rc;
+        const long long ncell = (long long)c->nx * c->ny * c->nz;
+        const long long nt = (ncell + kTile - 1) / kTile;
+        Scratch s;
+        long long *tcnt, *toff, *dtot;
+        if ((rc = s.get(c, &tcnt, nt)) || (rc = s.get(c, &toff, nt)) || (rc = s.get(c, &dtot, 1)))
+            return rc;
+        k_pec_faces<false><<<(unsigned)nt, kThreads, 0, c->stream>>>(
+            c->mat, c->nx, c->ny, c->nz, c->nxy, ncell, tcnt, nullptr, c->Pp, nullptr, nullptr);
+        HIPCHK(c, hipGetLastError());
+        if ((rc = scan_i64(c, s, tcnt, toff, nt, dtot)) || (rc = read_total(c, dtot, &n)))
+            return rc;
+        if (n > 0x7fffffffLL)
+            return fail(c, TLM_ERR_UNSUPPORTED, "more than 2^31 faces");
+        if ((rc = dalloc(c, &fc, (size_t)n)) || (rc = dalloc(c, &fa, (size_t)n)))
+        {
+            dfree(c, fc);
+            dfree(c, fa);
+            return rc;
+        }
+        k_pec_faces<true><<<(unsigned)nt, kThreads, 0, c->stream>>>(
+            c->mat, c->nx, c->ny, c->nz, c->nxy, ncell, nullptr, toff, c->Pp, fc, fa);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     // install as tlm_set_faces does
     dfree(c, c->fcell);

@@ -85,7 +85,17 @@ def test_each_voxeliser_alone():
     o = np.array(origin)
     sv, si = scenes.icosphere(0.25 * L.min(), 3)
     extra = np.vstack([sv, (o + 0.95 * L).astype(np.float32)[None]])
+    # ten slabs stacked along x in one mesh: rows cross it 20 times, more
+    # than the one-pass fill keeps per row (the two-pass fill takes over)
+    lv, li = [], []
+    for q in range(10):
+        v, ix = scenes.box_mesh(o + L * np.array([0.05 + 0.09 * q, 0.1, 0.1]),
+                                o + L * np.array([0.09 + 0.09 * q, 0.9, 0.8]))
+        li.append(ix + 8 * q)
+        lv.append(v)
+    layered = np.vstack(lv).astype(np.float32), np.vstack(li).astype(np.int32)
     cases = {
+        "layered solid": ("mesh", layered[0], layered[1], (0.0, 0.0, 0.0), 8, True),
         "sphere solid": ("mesh", sv, si, tuple(np.float32(o + 0.5 * L)), 3, True),
         "sphere surface": ("mesh", sv, si, tuple(np.float32(o + 0.5 * L)), 3, False),
         "unused vertex": ("mesh", extra, si, tuple(np.float32(o + 0.4 * L)), 4, True),
