@@ -343,6 +343,44 @@ def setup_run(n=256, subdiv=5):
     return out
 
 
+def ntff_run(n=256, offset=4, n_theta=37, n_phi=73, cpu_every=32):
+    """Near-to-far-field transform of the Huygens box of an n^3 grid (box
+    inset `offset` cells: 6 (n - 2 offset)^2 faces, seeded DFT values) on the
+    default 37 x 73 grid: device wall time (host face arrays in, U and the
+    directivity out) against the oracle's restatement of
+    ComputeFarFieldFromHuygens on one CPU thread over every cpu_every-th
+    face, scaled to all faces (the sums are linear in the face count)."""
+    from emsolver_amd import tlm
+    from oracle import oracle_tlm as ot
+    mat = np.zeros(n ** 3, np.uint8)
+    cells, axes, _ = tlm.huygens_faces(n, n, n, mat, offset)
+    dl = float(np.float32(0.0149896))
+    pos, sg = ot.huygens_geometry((n, n, n), (0.0, 0.0, 0.0), dl, cells, axes)
+    rng = np.random.default_rng(2)
+    dft = rng.normal(size=(cells.size, 8)).astype(np.float32)
+    best = None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        r = tlm.far_field(pos, axes, sg, dft, 1e9, n_theta, n_phi)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    sub = slice(0, None, cpu_every)
+    t0 = time.perf_counter()
+    ot.far_field(pos[sub], axes[sub], sg[sub], dft[sub], 1e9, n_theta, n_phi)
+    t_cpu = (time.perf_counter() - t0) * cpu_every
+    pairs = cells.size * n_theta * n_phi
+    out = {"faces": int(cells.size), "directions": n_theta * n_phi,
+           "device_ms": round(best * 1e3, 2),
+           "face_directions_per_s": round(pairs / best, 1),
+           "cpu_1thread_ms_est": round(t_cpu * 1e3, 1),
+           "cpu_sample": f"oracle float sums over every {cpu_every}th face, scaled",
+           "speedup_vs_1thread": round(t_cpu / best, 1),
+           "directivity": round(r.directivity, 5)}
+    log(f"[bench] ntff_{n}: {cells.size} faces x {n_theta * n_phi} directions: device "
+        f"{out['device_ms']} ms, CPU 1 thread ~{out['cpu_1thread_ms_est']:.0f} ms")
+    return out
+
+
 def cpu_baseline(scene, budget_s=15.0):
     from tests import helpers
     o = helpers.oracle_open(scene.with_steps(10**6))
@@ -382,7 +420,7 @@ def main():
     ap.add_argument("--lines", default="",
                     help="comma list of extra lines to run (diel_boxes_512_f32, diel_boxes_512_f16, "
                          "diel_boxes_512_bf16, free_space_512_f32, scene_setup_256, "
-                         "scene_setup_512, batched_256, served); default all")
+                         "scene_setup_512, ntff_256, batched_256, served); default all")
     ap.add_argument("--no-graph", action="store_true",
                     help="plain launches instead of captured HIP graphs in the batched lines "
                          "(for tracing tools that do not follow graph launches)")
@@ -473,6 +511,8 @@ def main():
     for sn in (256, 512):
         if not args.no_extra and want(f"scene_setup_{sn}"):
             extra[f"scene_setup_{sn}"] = setup_run(sn)
+    if not args.no_extra and want("ntff_256"):
+        extra["ntff_256"] = ntff_run(256)
     if not args.no_batch and want("batched_256"):
         extra["batched_256"] = batched_run(256)
     if not args.no_batch and want("served"):

@@ -37,6 +37,11 @@ class VoxelGrid(C.Structure):
     _fields_ = [("origin", C.c_float * 3), ("dl", C.c_float)]
 
 
+class FarFieldSummary(C.Structure):
+    _fields_ = [("u_max", C.c_float), ("directivity", C.c_float),
+                ("peak_theta_deg", C.c_float), ("peak_phi_deg", C.c_float)]
+
+
 class Tuning(C.Structure):
     _fields_ = [("kernel", C.c_int32), ("planes", C.c_int32), ("block_waves", C.c_int32),
                 ("nontemporal", C.c_int32), ("grid_blocks", C.c_int32)]
@@ -116,6 +121,10 @@ SIGNATURES = {
     "tlm_huygens_mat": (C.c_int64, [P, C.c_int]),
     "tlm_read_faces": (C.c_int64, [P, i64p, i32p, C.c_int64]),
     "tlm_read_huygens_faces": (C.c_int64, [P, i64p, i32p, C.c_int64]),
+    "tlm_far_field": (C.c_int, [C.c_int, C.c_int, fp, i32p, i32p, fp, C.c_int, C.c_float,
+                                C.c_int, C.c_int, fp, C.POINTER(FarFieldSummary)]),
+    "tlm_far_field_ctx": (C.c_int, [P, fp, C.c_int, C.c_float, C.c_int, C.c_int, fp,
+                                    C.POINTER(FarFieldSummary)]),
 }
 
 

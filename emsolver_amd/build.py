@@ -30,7 +30,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
           f"-I{os.path.join(HERE, 'host')}"]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-HIP_SOURCES = ["csrc/GpuTlmHip.hip", "csrc/TlmBatch.hip", "csrc/TlmVoxel.hip"]
+HIP_SOURCES = ["csrc/GpuTlmHip.hip", "csrc/TlmBatch.hip", "csrc/TlmVoxel.hip",
+               "csrc/TlmFarField.hip"]
 HOST_SOURCES = ["host/TlmSolver.cpp", "host/GpuTlm.cpp"]
 
 

@@ -301,13 +301,50 @@ float TlmSolver::DielScatterUnitarityError(float epsr)
 //   U ~ |L_phi + N_theta|^2 + |L_theta - N_phi|^2,
 // and the directivity 4*pi*Umax / integral(U sin(theta)) by the trapezoid
 // rule on the (theta, phi) grid.
+//
+// With a HIP device present the sums run on it (tlm_far_field); the loops
+// below are the host path of a machine without one, as the reference's CPU
+// loop is the path when RunGpuTlm returns false.
 //---------------------------------------------------------------------------
+static bool FarFieldOnDevice(const std::vector<HuyFace> &huyFaces, const std::vector<float> &huyDft,
+                             int huySamples, float f0, FarFieldData &out, int nTheta, int nPhi)
+{
+    const size_t nh = huyFaces.size();
+    std::vector<float> pos(3 * nh);
+    std::vector<int32_t> ax(nh), sg(nh);
+    for (size_t f = 0; f < nh; ++f)
+    {
+        pos[3 * f] = huyFaces[f].pos.x;
+        pos[3 * f + 1] = huyFaces[f].pos.y;
+        pos[3 * f + 2] = huyFaces[f].pos.z;
+        ax[f] = huyFaces[f].axis;
+        sg[f] = huyFaces[f].outSign;
+    }
+    std::vector<float> U((size_t)nTheta * nPhi);
+    tlm_far_field_summary sm;
+    if (tlm_far_field(0, (int)nh, pos.data(), ax.data(), sg.data(), huyDft.data(), huySamples, f0,
+                      nTheta, nPhi, U.data(), &sm) != TLM_OK)
+        return false;
+    out.nTheta = nTheta;
+    out.nPhi = nPhi;
+    out.U.swap(U);
+    out.uMax = sm.u_max;
+    out.directivity = sm.directivity;
+    out.peakThetaDeg = sm.peak_theta_deg;
+    out.peakPhiDeg = sm.peak_phi_deg;
+    return true;
+}
+
 bool ComputeFarFieldFromHuygens(const std::vector<HuyFace> &huyFaces,
                                 const std::vector<float> &huyDft, int huySamples, float f0,
                                 FarFieldData &out, int nTheta, int nPhi)
 {
     if (huyFaces.empty() || huySamples == 0 || nTheta < 2 || nPhi < 2)
         return false;
+    if (huyDft.size() < huyFaces.size() * 8)
+        return false;
+    if (tlm_device_count() > 0)
+        return FarFieldOnDevice(huyFaces, huyDft, huySamples, f0, out, nTheta, nPhi);
     const float PI = (float)M_PI;
     const float k0 = 2.0f * PI * f0 / kC0;
     const size_t nh = huyFaces.size();

@@ -741,3 +741,42 @@ def run_scenes_batched(scene_list, device=0, tuning=None, autotune=0, block=64):
     for d in devs:
         d.close()
     return out
+
+
+@dataclasses.dataclass
+class FarField:
+    U: np.ndarray            # [n_theta, n_phi]
+    u_max: float
+    directivity: float
+    peak_theta_deg: float
+    peak_phi_deg: float
+
+
+def far_field(pos, axis, sign, dft8, f0, n_theta=37, n_phi=73, samples=1, device=0) -> FarField:
+    """ComputeFarFieldFromHuygens on the device from host face arrays."""
+    lib = _native.load()
+    p = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    a = np.ascontiguousarray(axis, np.int32).reshape(-1)
+    s = np.ascontiguousarray(sign, np.int32).reshape(-1)
+    d = np.ascontiguousarray(dft8, np.float32).reshape(-1, 8)
+    U = np.zeros(n_theta * n_phi, np.float32)
+    sm = _native.FarFieldSummary()
+    rc = lib.tlm_far_field(int(device), len(p), ptr(p, C.c_float), ptr(a, C.c_int32),
+                           ptr(s, C.c_int32), ptr(d, C.c_float), int(samples), float(f0),
+                           n_theta, n_phi, ptr(U, C.c_float), C.byref(sm))
+    if rc != 0:
+        raise TlmError(f"tlm_far_field failed ({rc}): {lib.tlm_last_error(None).decode()}")
+    return FarField(U.reshape(n_theta, n_phi), sm.u_max, sm.directivity, sm.peak_theta_deg,
+                    sm.peak_phi_deg)
+
+
+def far_field_ctx(dev: TlmDevice, origin, samples, f0, n_theta=37, n_phi=73) -> FarField:
+    """The same from a context's resident Huygens accumulators."""
+    o = np.ascontiguousarray(np.asarray(origin, np.float32).reshape(3))
+    U = np.zeros(n_theta * n_phi, np.float32)
+    sm = _native.FarFieldSummary()
+    dev._chk(dev.lib.tlm_far_field_ctx(dev.h, ptr(o, C.c_float), int(samples), float(f0),
+                                       n_theta, n_phi, ptr(U, C.c_float), C.byref(sm)),
+             "tlm_far_field_ctx")
+    return FarField(U.reshape(n_theta, n_phi), sm.u_max, sm.directivity, sm.peak_theta_deg,
+                    sm.peak_phi_deg)

@@ -367,6 +367,37 @@ int64_t tlm_huygens_mat(tlm_ctx *ctx, int offset);
 int64_t tlm_read_faces(tlm_ctx *ctx, int64_t *cells, int32_t *axes, int64_t cap);
 int64_t tlm_read_huygens_faces(tlm_ctx *ctx, int64_t *cells, int32_t *axes, int64_t cap);
 
+/* ---------------------------------------------------------------------
+ * Near-to-far-field transform (ComputeFarFieldFromHuygens,
+ * TlmSolver.cpp:805-916) on the device: the O(directions x faces) phase sums
+ * of the Huygens box currents at f0, the intensity U on an
+ * n_theta x n_phi grid (theta-major, theta in [0, pi], phi in [0, 2 pi]),
+ * and the directivity / peak of FarFieldData (TlmSolver.cpp:893-913).
+ * U agrees with the reference's float sums to rounding (the projections
+ * are applied after the sums and the partial sums are added in double),
+ * not bit for bit.
+ * ------------------------------------------------------------------- */
+typedef struct tlm_far_field_summary {
+    float u_max;          /* FarFieldData::uMax */
+    float directivity;    /* 4 pi uMax / trapezoid integral of U sin(theta) */
+    float peak_theta_deg; /* FarFieldData::peakThetaDeg */
+    float peak_phi_deg;   /* FarFieldData::peakPhiDeg */
+} tlm_far_field_summary;
+
+/* From host arrays, as the reference's function takes them: per face the
+ * HuyFace pos (3 floats), axis and outSign, and the 8 DFT floats of huyDft
+ * (E1, E2, eta0 H1, eta0 H2 as re, im). U may be NULL. TLM_ERR_STATE when
+ * nfaces or samples is 0 (the reference returns false). */
+int tlm_far_field(int device, int nfaces, const float *pos, const int32_t *axis,
+                  const int32_t *out_sign, const float *dft8, int samples, float f0, int n_theta,
+                  int n_phi, float *U, tlm_far_field_summary *out);
+/* From a context's own Huygens faces and accumulators (no readback of the
+ * DFTs): positions cellCenter - domain centre for the given grid origin
+ * (TlmSolver.cpp:323-334), outward signs from the side of the grid the face
+ * is on. samples = the DFT sample count (0: no data). */
+int tlm_far_field_ctx(tlm_ctx *ctx, const float origin[3], int samples, float f0, int n_theta,
+                      int n_phi, float *U, tlm_far_field_summary *out);
+
 #ifdef __cplusplus
 }
 #endif

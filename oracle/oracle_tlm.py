@@ -133,6 +133,10 @@ def load_library():
         "ot_box_mesh": (None, [fp, fp, fp, i32p]),
         "ot_carve_feed": (C.c_int, [C.c_int, C.c_int, C.c_int, fp, C.c_float, fp, fp, fp,
                                     C.POINTER(C.c_uint8), i64p, C.c_int, i32p]),
+        "ot_far_field": (C.c_int, [C.c_int, fp, i32p, i32p, fp, C.c_int, C.c_float, C.c_int,
+                                   C.c_int, C.c_int, fp, fp]),
+        "ot_huygens_geometry": (None, [C.c_int, C.c_int, C.c_int, fp, C.c_float, C.c_int, i64p,
+                                       i32p, fp, i32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -475,3 +479,37 @@ def carve_feed(mat, dims, origin, dl, a, b, offset=(0, 0, 0)):
                                      _ptr(off, C.c_float), _ptr(mat, C.c_uint8),
                                      _ptr(cells, C.c_int64), cap, C.byref(pol))
     return cells[:min(n, cap)].copy(), int(pol.value)
+
+
+# --- near-to-far-field transform (TlmSolver.cpp:805-916) --------------------
+
+def far_field(pos, axis, sign, dft8, f0, n_theta=37, n_phi=73, samples=1, fp64=False):
+    """ComputeFarFieldFromHuygens: (U[n_theta, n_phi], uMax, directivity,
+    peak theta deg, peak phi deg), or None without data."""
+    p = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    a = np.ascontiguousarray(axis, np.int32).reshape(-1)
+    s = np.ascontiguousarray(sign, np.int32).reshape(-1)
+    d = np.ascontiguousarray(dft8, np.float32).reshape(-1, 8)
+    U = np.zeros(n_theta * n_phi, np.float32)
+    summ = np.zeros(4, np.float32)
+    rc = load_library().ot_far_field(len(p), _ptr(p, C.c_float), _ptr(a, C.c_int32),
+                                     _ptr(s, C.c_int32), _ptr(d, C.c_float), int(samples),
+                                     C.c_float(f0), n_theta, n_phi, int(bool(fp64)),
+                                     _ptr(U, C.c_float), _ptr(summ, C.c_float))
+    if rc != 0:
+        return None
+    return U.reshape(n_theta, n_phi), float(summ[0]), float(summ[1]), float(summ[2]), \
+        float(summ[3])
+
+
+def huygens_geometry(dims, origin, dl, cells, axis):
+    """HuyFace pos and outSign of Huygens cells (TlmSolver.cpp:323-334)."""
+    nx, ny, nz = dims
+    c = np.ascontiguousarray(cells, np.int64)
+    a = np.ascontiguousarray(axis, np.int32)
+    pos = np.zeros((c.size, 3), np.float32)
+    sg = np.zeros(c.size, np.int32)
+    load_library().ot_huygens_geometry(nx, ny, nz, _ptr(_f3(origin), C.c_float), C.c_float(dl),
+                                       c.size, _ptr(c, C.c_int64), _ptr(a, C.c_int32),
+                                       _ptr(pos, C.c_float), _ptr(sg, C.c_int32))
+    return pos, sg
