@@ -179,19 +179,23 @@ def batched_run(count=256, sample_every=4):
     return out
 
 
-def served_run(count=200, rate=50.0, baseline_sample=20, seed=5, max_cells=int(2.5e8)):
+def served_run(count=200, rate=50.0, baseline_sample=20, seed=5, max_cells=int(2.5e8),
+               device_setup=False):
     """The served mixed-size stream (Poisson arrivals at `rate`/s, axes
     log-uniform in [32,384], steps U[500,4000], FP16 storage for the requests
     that tolerate 1e-3, FP32 otherwise) through the resident serving loop.
-    Baseline: `baseline_sample` of the same requests solved one at a time as
-    plain FP32 contexts."""
+    device_setup: each request's scene is voxelised on the device at
+    admission instead of built and uploaded from the host. Baseline:
+    `baseline_sample` of the same requests solved one at a time as plain FP32
+    contexts."""
     from emsolver_amd import serve, tlm
-    reqs = serve.served_stream(count=count, rate=rate, seed=seed)
+    reqs = serve.served_stream(count=count, rate=rate, seed=seed, device_setup=device_setup)
     srv = serve.TlmServer(block_ticks=32, autotune_ticks=2, keep_results=False, progress_s=20.0,
                           max_cells=max_cells, tuning={"graph_ticks": GRAPH_TICKS})
     done = srv.serve(reqs)
     summ = serve.summarize(done)
     summ.update({"rate_per_s": rate, "mean_members": round(srv.stats["mean_members"], 2),
+                 "scene_setup": "device voxelisers" if device_setup else "host map + upload",
                  "tuning": srv.stats["tuning"], "admission": f"FIFO, free HBM, <= {max_cells} cells in flight"})
     alone_work, alone_wall = 0, 0.0
     for r in reqs[:baseline_sample]:
@@ -428,6 +432,9 @@ def main():
                     help="only report the 16-bit storage error against the FP64 oracle (JSON)")
     ap.add_argument("--serve-only", action="store_true",
                     help="only run the served stream (--serve-count requests) and print its JSON")
+    ap.add_argument("--serve-host-setup", action="store_true",
+                    help="served requests as host material maps uploaded at admission (default: "
+                         "set-up operations voxelised on the device)")
     ap.add_argument("--serve-count", type=int, default=200,
                     help="requests of the served line (the full stream is 2000)")
     args = ap.parse_args()
@@ -449,7 +456,8 @@ def main():
         print(json.dumps(precision_report()), flush=True)
         return
     if args.serve_only:
-        print(json.dumps(served_run(args.serve_count)), flush=True)
+        print(json.dumps(served_run(args.serve_count, device_setup=not args.serve_host_setup)),
+              flush=True)
         return
     dev_name = tlm.device_name(0)
     log(f"[bench] device {dev_name}")
@@ -516,7 +524,8 @@ def main():
     if not args.no_batch and want("batched_256"):
         extra["batched_256"] = batched_run(256)
     if not args.no_batch and want("served"):
-        extra[f"served_{args.serve_count}"] = served_run(args.serve_count)
+        extra[f"served_{args.serve_count}"] = served_run(args.serve_count,
+                                                         device_setup=not args.serve_host_setup)
 
     cpu = None
     if not args.no_cpu:
