@@ -527,50 +527,40 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
             const float R0 = __shfl_down(o[0], 1, kWave), R1 = __shfl_down(o[1], 1, kWave);
             if (act)
             {
-                // ---- x faces: pair 0 (x-: ports 0,1), pair 1 (x+: ports 2,3) ----
-                const bool pecL = (code & NB_XM) != 0u, pecR = (code & NB_XP) != 0u;
-                if (pec)
-                    stpair<PF, H>(Bk, bo, s, bx0 ? rho * o[0] : o[0], bx0 ? rho * o[1] : o[1]);
-                else if (bx0)
-                    stpair<PF, H>(Bk, bo, s, rho * o[0], rho * o[1]);
-                else if (pecL)
-                    stpair<PF, H>(Bk, bo, s, -o[0], -o[1]);
-                else if (hasL)
-                    stpair<PF, H>(Bk, bo, s, L0, L1);
-                else
-                    stpair<PF, H>(Bk + PQ, blk16((unsigned)f - 1u, ES), s, o[0], o[1]);
-                if (pec)
-                    stpair<PF, H>(Bk + PQ, bo, s, bx1 ? rho * o[2] : o[2], bx1 ? rho * o[3] : o[3]);
-                else if (bx1)
-                    stpair<PF, H>(Bk + PQ, bo, s, rho * o[2], rho * o[3]);
-                else if (pecR)
-                    stpair<PF, H>(Bk + PQ, bo, s, -o[2], -o[3]);
-                else if (hasR)
-                    stpair<PF, H>(Bk + PQ, bo, s, R0, R1);
-                else
-                    stpair<PF, H>(Bk, blk16((unsigned)f + 1u, ES), s, o[2], o[3]);
-                // ---- y faces: pair 2 (y-: ports 4,5), pair 3 (y+: ports 6,7) ----
-                const bool pecLo = (code & NB_YM) != 0u, pecHi = (code & NB_YP) != 0u;
-                if (pec)
-                {
-                    stpair<PF, H>(Bk + 2 * PQ, bo, s, by0 ? rho * o[4] : o[4], by0 ? rho * o[5] : o[5]);
-                    stpair<PF, H>(Bk + 3 * PQ, bo, s, by1 ? rho * o[6] : o[6], by1 ? rho * o[7] : o[7]);
-                }
-                else
-                {
-                    if (by0)
-                        stpair<PF, H>(Bk + 2 * PQ, bo, s, rho * o[4], rho * o[5]);
-                    else if (pecLo)
-                        stpair<PF, H>(Bk + 2 * PQ, bo, s, -o[4], -o[5]);
-                    else
-                        stpair<PF, H>(Bk + 3 * PQ, blk16((unsigned)(f - a.nx), ES), s, o[4], o[5]);
-                    if (by1)
-                        stpair<PF, H>(Bk + 3 * PQ, bo, s, rho * o[6], rho * o[7]);
-                    else if (pecHi)
-                        stpair<PF, H>(Bk + 3 * PQ, bo, s, -o[6], -o[7]);
-                    else
-                        stpair<PF, H>(Bk + 2 * PQ, blk16((unsigned)(f + a.nx), ES), s, o[6], o[7]);
-                }
+                // x and y faces: the value and the slot (this cell's own face,
+                // or the neighbour's opposite face) are selected per lane, so
+                // every face is one encode and one store
+                auto face = [&](int p, bool bnd, bool pecN, bool mine, float n0, float n1,
+                                unsigned own_q, unsigned nb_q, unsigned nb_g) {
+                    float x = o[p], y = o[p + 1];
+                    if (bnd)
+                    {
+                        x = rho * o[p];
+                        y = rho * o[p + 1];
+                    }
+                    else if (!pec && pecN)
+                    {
+                        x = -o[p];
+                        y = -o[p + 1];
+                    }
+                    else if (!pec && mine)
+                    {
+                        x = n0;
+                        y = n1;
+                    }
+                    const bool own = pec || bnd || pecN || mine;
+                    const unsigned off = own ? own_q * PQ + bo : nb_q * PQ + blk16(nb_g, ES);
+                    stpair<PF, H>(Bk, off, s, x, y);
+                };
+                // pair 0 (x-: ports 0,1), pair 1 (x+: ports 2,3): the left /
+                // right neighbour in this wave takes the pulse from a shuffle
+                face(0, bx0, (code & NB_XM) != 0u, hasL, L0, L1, 0u, 1u, (unsigned)f - 1u);
+                face(2, bx1, (code & NB_XP) != 0u, hasR, R0, R1, 1u, 0u, (unsigned)f + 1u);
+                // pair 2 (y-: ports 4,5), pair 3 (y+: ports 6,7): always pushed
+                face(4, by0, (code & NB_YM) != 0u, false, 0.f, 0.f, 2u, 3u,
+                     (unsigned)(f - a.nx));
+                face(6, by1, (code & NB_YP) != 0u, false, 0.f, 0.f, 3u, 2u,
+                     (unsigned)(f + a.nx));
                 // ---- z faces: pair 4 (z-: ports 8,9), pair 5 (z+: ports 10,11) ----
                 if (k > k0)
                 {
@@ -636,7 +626,7 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int wpb = blockDim.x / kWave;
-    const int wave = threadIdx.x / kWave;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const XcdRange gr = xcd_range(blockIdx.x, gridDim.x, a.groups);
     double esum = 0.0;
     for (int g = gr.first; g < gr.end; g += gr.step)
