@@ -4,7 +4,10 @@ arrive over time, are admitted against free HBM into one running batch
 the others keep running), and complete with their probe and port records.
 
 The device work is the batched engine of the C ABI (``tlm_batch_*``): a block
-of ticks is one captured HIP graph. A member's result is the same bits alone
+of ticks is one captured HIP graph. While a block runs, the host reads the
+records of the members that left after the previous block and sets up the
+next admitted requests (their contexts run their set-up on their own
+streams), so the device does not wait for that host work. A member's result is the same bits alone
 and in the mix (each grid's arithmetic does not depend on its neighbours in
 the batch); tests/test_gpu_serve.py checks exactly that.
 """
@@ -119,13 +122,48 @@ class TlmServer:
         now = (lambda: time.perf_counter() - t0) if realtime else (lambda: 0.0)
         arrivals = collections.deque(reqs)
         queue = collections.deque()
-        live = {}       # TlmDevice -> (request, scene, admitted time)
+        live = {}       # TlmDevice -> (request, scene, admitted time): in the batch
+        pending = []    # [(TlmDevice, request, scene, admitted)]: set up, joining next block
+        finished = []   # [(TlmDevice, request, scene, admitted)]: left the batch, to read
         done = []
         tuned = False
         ticks = 0
         member_ticks = 0
         last_report = time.perf_counter()
-        while arrivals or queue or live:
+
+        def admit():
+            """Set up the queue's head requests that fit (FIFO, free HBM, cells
+            in flight); their contexts join the batch at the next block."""
+            while queue and len(live) + len(pending) < self.max_members:
+                r = queue[0]
+                sc = take(r.rid)
+                need = self._need_bytes(sc)
+                free, _ = tlm.device_memory(self.device)
+                cells = sum(v[1].ncell for v in live.values()) + sum(p[2].ncell for p in pending)
+                over = self.max_cells > 0 and cells + sc.ncell > self.max_cells
+                if (live or pending) and (need + self.hbm_reserve > free or over):
+                    with cond:  # put it back; try again after members leave
+                        ready[r.rid] = sc
+                        cond.notify_all()
+                    return
+                queue.popleft()
+                pending.append((tlm.open_scene(sc, self.device), r, sc, now()))
+
+        def complete():
+            """Read the records of the members that left and free them."""
+            for dev, r, sc, adm in finished:
+                c = Completion(r.rid, r.arrival, adm, 0.0, sc.ncell, sc.total_steps, sc.pulse)
+                if self.keep_results:
+                    if dev.nprobe:
+                        c.probes = dev.read_probes(0, sc.total_steps)
+                    if dev.nport:
+                        c.ports = dev.read_ports(0, sc.total_steps)
+                dev.close()
+                c.finished = now() if realtime else time.perf_counter() - t0
+                done.append(c)
+            finished.clear()
+
+        while arrivals or queue or live or pending or finished:
             if self.progress_s > 0 and time.perf_counter() - last_report >= self.progress_s:
                 last_report = time.perf_counter()
                 import sys
@@ -134,49 +172,32 @@ class TlmServer:
             t = now()
             while arrivals and arrivals[0].arrival <= t:
                 queue.append(arrivals.popleft())
-            # admission: FIFO, against free device memory
-            while queue and len(live) < self.max_members:
-                r = queue[0]
-                sc = take(r.rid)
-                need = self._need_bytes(sc)
-                free, _ = tlm.device_memory(self.device)
-                cells = sum(v[1].ncell for v in live.values())
-                over = self.max_cells > 0 and cells + sc.ncell > self.max_cells
-                if live and (need + self.hbm_reserve > free or over):
-                    with cond:  # put it back; try again after members leave
-                        ready[r.rid] = sc
-                        cond.notify_all()
-                    break
-                queue.popleft()
-                dev = tlm.open_scene(sc, self.device)
+            # the contexts set up during the last block join now
+            for dev, r, sc, adm in pending:
                 batch.add(dev, sc.total_steps)
-                live[dev] = (r, sc, now())
+                live[dev] = (r, sc, adm)
+            pending.clear()
+            if live:
+                if self.autotune_ticks and not tuned and len(live) >= 2:
+                    batch.autotune(self.autotune_ticks)
+                    tuned = True
+                batch.run(self.block_ticks)          # queued; the host goes on below
+                ticks += self.block_ticks
+                member_ticks += self.block_ticks * len(live)
+            # while the block runs: results out, new requests set up
+            complete()
+            admit()
             if not live:
-                if arrivals and realtime:
+                if not pending and arrivals and realtime:
                     time.sleep(max(0.0, min(0.05, arrivals[0].arrival - now())))
                 continue
-            if self.autotune_ticks and not tuned and len(live) >= 2:
-                batch.autotune(self.autotune_ticks)
-                tuned = True
-            batch.run(self.block_ticks)
-            ticks += self.block_ticks
-            member_ticks += self.block_ticks * len(live)
             batch.sync()
             for dev, left in batch.remaining().items():
                 if left > 0:
                     continue
                 r, sc, adm = live.pop(dev)
                 batch.remove(dev)
-                c = Completion(r.rid, r.arrival, adm, now(), sc.ncell, sc.total_steps, sc.pulse)
-                if self.keep_results:
-                    if dev.nprobe:
-                        c.probes = dev.read_probes(0, sc.total_steps)
-                    if dev.nport:
-                        c.ports = dev.read_ports(0, sc.total_steps)
-                dev.close()
-                if not realtime:
-                    c.finished = time.perf_counter() - t0
-                done.append(c)
+                finished.append((dev, r, sc, adm))
         self.stats = {"ticks": ticks, "mean_members": member_ticks / max(1, ticks),
                       "wall_s": time.perf_counter() - t0, "tuning": batch.get_tuning()}
         batch.close()
