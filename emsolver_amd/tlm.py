@@ -372,7 +372,7 @@ class TlmDevice:
                                           C.byref(ker) if kernel else None), "tlm_time_steps")
         return tot.value, (ker.value if kernel else None)
 
-    def set_tuning(self, kernel=0, planes=1, block_waves=0, nontemporal=0, grid_blocks=0):
+    def set_tuning(self, kernel=0, planes=1, block_waves=0, nontemporal=2, grid_blocks=0):
         t = _native.Tuning(int(kernel), int(planes), int(block_waves), int(nontemporal),
                            int(grid_blocks))
         self._chk(self.lib.tlm_set_tuning(self.h, C.byref(t)), "tlm_set_tuning")

@@ -212,7 +212,8 @@ typedef struct tlm_tuning {
     int32_t kernel;      /* 0: one cell per lane, 64-cell strips per wavefront (only variant) */
     int32_t planes;      /* z slabs marched per wave task (default 1); 0 = automatic */
     int32_t block_waves; /* wavefronts per workgroup, 1..4; 0 = 4 */
-    int32_t nontemporal; /* 1: non-temporal stores of the next-step pulses (default 0) */
+    int32_t nontemporal; /* non-temporal stores of the next-step pulses: 0 no, 1 yes, 2 when the
+                            grid has dielectric cells (default) */
     int32_t grid_blocks; /* > 0: persistent grid of that many workgroups; 0 = one per task group */
 } tlm_tuning;
 

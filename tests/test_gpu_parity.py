@@ -120,6 +120,7 @@ TUNINGS = [
     {"kernel": 0, "planes": 3, "block_waves": 1, "nontemporal": 0, "grid_blocks": 5},
     {"kernel": 0, "planes": 1, "block_waves": 2, "nontemporal": 1, "grid_blocks": 17},
     {"kernel": 0, "planes": 64, "block_waves": 4, "nontemporal": 0, "grid_blocks": 8},
+    {"kernel": 0, "planes": 2, "block_waves": 4, "nontemporal": 2, "grid_blocks": 0},
 ]
 
 
