@@ -193,3 +193,25 @@ def test_set_up_errors():
         dev.upload_mat([0.0, 0.0, 0.0, 4.0], [0.0] * 4)
         m = dev.mat_read()
         assert (m == 5).sum() == 8 and set(np.unique(m)) == {0, 5}
+
+
+def test_degenerate_inputs_match_oracle():
+    """Empty meshes and wires, a single triangle asked for a solid (surface
+    only: no ray pair), a feed outside the grid, a box clipped away."""
+    dims, origin, dl = (20, 18, 16), (0.0, 0.0, 0.0), 0.05
+    tri = np.float32([[0.2, 0.2, 0.3], [0.7, 0.3, 0.35], [0.4, 0.75, 0.4]])
+    sc = scenes.Scene(*dims, None, dl=dl, origin=origin)
+    sc.ops = [("mesh", np.zeros((0, 3), np.float32), np.zeros((0, 3), np.int32), (0, 0, 0), 3, True),
+              ("mesh", tri, np.int32([[0, 1, 2]]), (0, 0, 0), 4, True),
+              ("wire", np.float32([[0.3, 0.3, 0.3]]), 0.1, (0, 0, 0), 5),
+              ("wire", np.zeros((0, 3), np.float32), 0.0, (0, 0, 0), 5),
+              ("cells_box", (30, 0, 0), (40, 5, 5), 6),
+              ("cells_box", (-5, -5, -5), (2, 2, 2), 7),
+              ("feed", (5.0, 5.0, 5.0), (5.0, 5.0, 5.2), (0, 0, 0), 1.0)]
+    host = helpers.oracle_build(sc)
+    with tlm.TlmDevice(*dims, dl) as dev:
+        feeds = tlm.apply_ops(dev, sc)
+        got = dev.mat_read()
+    assert feeds == [] and host.ports == []
+    assert set(np.unique(host.mat)) == {0, 4, 7}
+    assert np.array_equal(got, host.mat)
