@@ -69,29 +69,34 @@ size_t energy_parts(const tlm_ctx *c)
     return (size_t)c->blocks * c->wpb;
 }
 
+// The fused step launch; nmon > 0 appends the workgroups that take the
+// step's monitor records.
 template <bool S, bool E, int PF>
-void launch_fused(tlm_ctx *c, const tlm::StepArgs &a)
+void launch_fused(tlm_ctx *c, const tlm::StepArgs &a, const tlm::MonitorArgs &m, int nmon)
 {
-    dim3 grid(c->blocks), block(c->wpb * tlm::kWave);
+    const int bt = c->wpb * tlm::kWave;
+    const int mon_blocks = nmon > 0 ? (nmon + bt - 1) / bt : 0;
+    dim3 grid(c->blocks + mon_blocks), block(bt);
     // automatic: non-temporal pulse stores when the grid has stub cells
     // (faster there by 1-3 %; slower by 3-4 % in plain air)
     const bool nt = c->tune.nontemporal == 1 || (c->tune.nontemporal == 2 && c->stubs);
     if (nt && PF == tlm::PF_F32)
-        hipLaunchKernelGGL((tlm::k_step<S, E, true, PF>), grid, block, 0, c->stream, a);
+        hipLaunchKernelGGL((tlm::k_step<S, E, true, PF>), grid, block, 0, c->stream, a, m, c->blocks);
     else
-        hipLaunchKernelGGL((tlm::k_step<S, E, false, PF>), grid, block, 0, c->stream, a);
+        hipLaunchKernelGGL((tlm::k_step<S, E, false, PF>), grid, block, 0, c->stream, a, m, c->blocks);
 }
 
-template <int PF> void launch_pf(tlm_ctx *c, const tlm::StepArgs &a, bool energy)
+template <int PF>
+void launch_pf(tlm_ctx *c, const tlm::StepArgs &a, bool energy, const tlm::MonitorArgs &m, int nmon)
 {
     if (c->stubs)
-        energy ? launch_fused<true, true, PF>(c, a) : launch_fused<true, false, PF>(c, a);
+        energy ? launch_fused<true, true, PF>(c, a, m, nmon) : launch_fused<true, false, PF>(c, a, m, nmon);
     else
-        energy ? launch_fused<false, true, PF>(c, a) : launch_fused<false, false, PF>(c, a);
+        energy ? launch_fused<false, true, PF>(c, a, m, nmon) : launch_fused<false, false, PF>(c, a, m, nmon);
 }
 
 
-int launch_step(tlm_ctx *c, int n)
+int launch_step(tlm_ctx *c, int n, const tlm::MonitorArgs &m, int nmon)
 {
     const bool energy = c->every > 0 && (n % c->every) == 0;
     tlm::StepArgs a;
@@ -128,11 +133,11 @@ int launch_step(tlm_ctx *c, int n)
     a.groups = c->groups;
     a.rho = c->rho;
     if (c->pf == TLM_PULSE_F16)
-        launch_pf<tlm::PF_F16>(c, a, energy);
+        launch_pf<tlm::PF_F16>(c, a, energy, m, nmon);
     else if (c->pf == TLM_PULSE_BF16)
-        launch_pf<tlm::PF_BF16>(c, a, energy);
+        launch_pf<tlm::PF_BF16>(c, a, energy, m, nmon);
     else
-        launch_pf<tlm::PF_F32>(c, a, energy);
+        launch_pf<tlm::PF_F32>(c, a, energy, m, nmon);
     if (c->pf != TLM_PULSE_F32)
         hipLaunchKernelGGL(tlm::k_slab_max, dim3(c->nz), dim3(256), 0, c->stream, c->mpart,
                            c->strips, c->mreg);
@@ -236,9 +241,9 @@ int queue_steps(tlm_ctx *c, int step0, int nsteps, bool time_kernel, float *ms_k
         const bool plane_now = c->pa >= 0 && (n % 2) == 0;
         const int pn = plane_now ? c->pn1 * c->pn2 : 0;
         const int nmon = nmon0 + pn;
+        tlm::MonitorArgs m = {};
         if (nmon > 0)
         {
-            tlm::MonitorArgs m;
             m.pn = pn;
             m.pa = c->pa;
             m.pidx = c->pidx;
@@ -273,11 +278,15 @@ int queue_steps(tlm_ctx *c, int step0, int nsteps, bool time_kernel, float *ms_k
             m.prcell = c->prcell;
             m.inv_dl = c->inv_dl;
             m.prrec = c->prrec;
-            int b = 256, g = (nmon + b - 1) / b;
-            hipLaunchKernelGGL(tlm::k_monitor, dim3(g), dim3(b), 0, c->stream, m);
             if (plane_now)
                 c->plane_step = n;
         }
+        // the monitor records of step n run in the step launch's extra workgroups
+        if (time_kernel)
+            HIPCHK(c, hipEventRecord(c->kev[2 * (n - step0)], c->stream));
+        launch_step(c, n, m, nmon);
+        if (time_kernel)
+            HIPCHK(c, hipEventRecord(c->kev[2 * (n - step0) + 1], c->stream));
         if (c->frame_every > 0 && (n % c->frame_every) == 0 &&
             (int)c->frame_steps.size() < c->frame_max)
         {
@@ -293,11 +302,6 @@ int queue_steps(tlm_ctx *c, int step0, int nsteps, bool time_kernel, float *ms_k
             c->frame_steps.push_back(n);
             c->frame_plane_step.push_back(c->pa >= 0 ? c->plane_step : -1);
         }
-        if (time_kernel)
-            HIPCHK(c, hipEventRecord(c->kev[2 * (n - step0)], c->stream));
-        launch_step(c, n);
-        if (time_kernel)
-            HIPCHK(c, hipEventRecord(c->kev[2 * (n - step0) + 1], c->stream));
         if (c->every > 0 && (n % c->every) == 0)
         {
             const int idx = n / c->every;

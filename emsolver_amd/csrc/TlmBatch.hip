@@ -253,12 +253,17 @@ int rebuild(tlm_batch *b)
     return TLM_OK;
 }
 
-template <bool S, int PF> void launch_kind(tlm_batch *b, int k)
+// One kind's step launch; with_mon appends the workgroups that take the
+// tick's probe and port records.
+template <bool S, int PF> void launch_kind(tlm_batch *b, int k, bool with_mon)
 {
     const int ng = b->ngroups[k];
     const int blocks = b->tune.grid_blocks > 0 ? std::min(ng, b->tune.grid_blocks) : ng;
-    hipLaunchKernelGGL((tlm::k_step_batch<S, PF>), dim3(blocks), dim3(b->wpb * tlm::kWave), 0,
-                       b->stream, b->d_mem, b->d_gmem[k], ng, b->d_tick);
+    const int bt = b->wpb * tlm::kWave;
+    const int mon_blocks = with_mon && b->nmon > 0 ? (b->nmon + bt - 1) / bt : 0;
+    hipLaunchKernelGGL((tlm::k_step_batch<S, PF>), dim3(blocks + mon_blocks), dim3(bt), 0,
+                       b->stream, b->d_mem, b->d_gmem[k], ng, b->d_tick, b->d_mon, b->nmon,
+                       blocks);
 }
 
 void queue_tick(tlm_batch *b)
@@ -269,24 +274,26 @@ void queue_tick(tlm_batch *b)
         hipLaunchKernelGGL(tlm::k_inject_batch, dim3((n + t - 1) / t), dim3(t), 0, b->stream,
                            b->d_mem, b->d_inj, b->ninj, b->nplan, b->d_tick);
     }
-    if (b->nmon > 0)
+    // the records ride with the first kind that has members
+    bool mon = true;
+    auto kind = [&](int k, auto launch) {
+        if (!b->ngroups[k])
+            return;
+        launch(k, mon);
+        mon = false;
+    };
+    kind(0, [&](int k, bool m) { launch_kind<false, tlm::PF_F32>(b, k, m); });
+    kind(1, [&](int k, bool m) { launch_kind<false, tlm::PF_F16>(b, k, m); });
+    kind(2, [&](int k, bool m) { launch_kind<false, tlm::PF_BF16>(b, k, m); });
+    kind(3, [&](int k, bool m) { launch_kind<true, tlm::PF_F32>(b, k, m); });
+    kind(4, [&](int k, bool m) { launch_kind<true, tlm::PF_F16>(b, k, m); });
+    kind(5, [&](int k, bool m) { launch_kind<true, tlm::PF_BF16>(b, k, m); });
+    if (mon && b->nmon > 0)
     {
-        const int t = 128;
-        hipLaunchKernelGGL(tlm::k_monitor_batch, dim3((b->nmon + t - 1) / t), dim3(t), 0,
-                           b->stream, b->d_mem, b->d_mon, b->nmon, b->d_tick);
+        // no member with step groups (cannot happen with active members):
+        // the records alone, through an empty step grid
+        launch_kind<false, tlm::PF_F32>(b, 0, true);
     }
-    if (b->ngroups[0])
-        launch_kind<false, tlm::PF_F32>(b, 0);
-    if (b->ngroups[1])
-        launch_kind<false, tlm::PF_F16>(b, 1);
-    if (b->ngroups[2])
-        launch_kind<false, tlm::PF_BF16>(b, 2);
-    if (b->ngroups[3])
-        launch_kind<true, tlm::PF_F32>(b, 3);
-    if (b->ngroups[4])
-        launch_kind<true, tlm::PF_F16>(b, 4);
-    if (b->ngroups[5])
-        launch_kind<true, tlm::PF_BF16>(b, 5);
     if (b->nsmax > 0)
         hipLaunchKernelGGL(tlm::k_slab_max_batch, dim3(b->nsmax), dim3(256), 0, b->stream,
                            b->d_mem, b->d_smax, b->d_tick);

@@ -4,10 +4,11 @@
 // Each member grid keeps its own context buffers (pulses, stubs, cell codes,
 // monitors). A tick advances every active member by one step: one
 // injection launch for all members' soft sources (with the 16-bit scale
-// planning), one monitor launch for all probes and ports, one fused step
-// launch per member kind (stubs x storage format) over the concatenated task
-// groups of its members, one slab-maximum launch for the 16-bit members and
-// a one-thread launch that advances the device tick. Every launch reads the
+// planning), one fused step launch per member kind (stubs x storage format)
+// over the concatenated task groups of its members (the first of them also
+// taking every member's probe and port records in extra workgroups), one
+// slab-maximum launch for the 16-bit members and a one-thread launch that
+// advances the device tick. Every launch reads the
 // tick from device memory, so a block of ticks can be captured once in a
 // HIP graph and replayed.
 #pragma once
@@ -112,10 +113,9 @@ static __global__ void k_inject_batch(const BatchMember *mem, const int2 *ent, i
 
 // Probe and port records of every member (entries (member, index): index <
 // npr is a probe, the rest ports).
-static __global__ void k_monitor_batch(const BatchMember *mem, const int2 *ent, int nent,
-                                const long long *tick_p)
+__device__ __forceinline__ void monitor_batch_item(const BatchMember *mem, const int2 *ent, int nent,
+                                                   const long long *tick_p, int id)
 {
-    const int id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= nent)
         return;
     const int2 e = ent[id];
@@ -133,15 +133,24 @@ static __global__ void k_monitor_batch(const BatchMember *mem, const int2 *ent, 
 // Fused scatter + connect for the members of one kind. gmem[g] is the member
 // of task group g; the workgroups walk the groups in XCD-contiguous ranges
 // (a persistent grid when there are fewer workgroups than groups).
+// Workgroups from mon_from on take the tick's probe and port records of all
+// members (they read the incident pulses, which the step only reads).
 template <bool STUBS, int PF>
 __global__ void __launch_bounds__(256) k_step_batch(const BatchMember *mem, const int *gmem,
-                                                    int ngroups, const long long *tick_p)
+                                                    int ngroups, const long long *tick_p,
+                                                    const int2 *ment, int nment, int mon_from)
 {
+    if ((int)blockIdx.x >= mon_from)
+    {
+        monitor_batch_item(mem, ment, nment, tick_p,
+                           ((int)blockIdx.x - mon_from) * (int)blockDim.x + (int)threadIdx.x);
+        return;
+    }
     const int lane = threadIdx.x & (kWave - 1);
     const int wpb = blockDim.x / kWave;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const long long tick = *tick_p;
-    const XcdRange gr = xcd_range(blockIdx.x, gridDim.x, ngroups);
+    const XcdRange gr = xcd_range(blockIdx.x, mon_from, ngroups);
     double esum = 0.0;
     for (int g = gr.first; g < gr.end; g += gr.step)
     {

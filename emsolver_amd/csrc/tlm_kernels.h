@@ -22,11 +22,13 @@
 //
 // One step is (TlmSolver::step, TlmSolver.cpp:485-513):
 //   k_inject  : soft sources add 0.5*amp*sv(n) into A        (:487-505)
-//   k_monitor : Js / Huygens / port / probe records from A     (:616-713)
 //   k_step    : scatter every cell from A and deliver each reflected
 //               pulse to the slot it reaches after connect, writing B
 //               (scatter :516-548 and connect :551-613 fused); optionally
-//               the FP64 sum of A^2 for the energy monitor (:766-789).
+//               the FP64 sum of A^2 for the energy monitor (:766-789);
+//               extra workgroups of the same launch take the Js /
+//               Huygens / port / probe records from A (monitor_item,
+//               :616-713), which the step itself only reads.
 #pragma once
 
 #include "scn_node.h"
@@ -621,33 +623,6 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
     }
 }
 
-template <bool STUBS, bool ENERGY, bool NT, int PF>
-__global__ void __launch_bounds__(256) k_step(StepArgs a)
-{
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wpb = blockDim.x / kWave;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const XcdRange gr = xcd_range(blockIdx.x, gridDim.x, a.groups);
-    double esum = 0.0;
-    for (int g = gr.first; g < gr.end; g += gr.step)
-    {
-        const int task = g * wpb + wave;
-        if (task < a.tasks)
-        {
-            if constexpr (PF == PF_F32)
-                strip_task<STUBS, ENERGY, NT, PF>(a, task, lane, esum);
-            else
-                strip_task16<STUBS, ENERGY, PF>(a, task, lane, esum);
-        }
-    }
-    if (ENERGY)
-    {
-        for (int off = kWave / 2; off > 0; off >>= 1)
-            esum += __shfl_down(esum, off, kWave);
-        if (lane == 0)
-            a.epart[blockIdx.x * wpb + wave] = esum;
-    }
-}
 
 // Per-slab maximum of the wave records of one step (one workgroup per slab).
 static __global__ void k_slab_max(const float *mpart, int strips, float *mreg)
@@ -921,9 +896,8 @@ struct MonitorArgs
 
 // Monitors on the post-injection incident pulses (TlmSolver::monitors,
 // TlmSolver.cpp:616-713), one thread per face / port / probe.
-static __global__ void k_monitor(MonitorArgs a)
+__device__ __forceinline__ void monitor_item(const MonitorArgs &a, int id)
 {
-    int id = blockIdx.x * blockDim.x + threadIdx.x;
     const bool doDft = a.n >= a.settle;
     if (id < a.nf)
     {
@@ -1003,6 +977,44 @@ static __global__ void k_monitor(MonitorArgs a)
             e = sqrtf(e2);
         }
         a.plane[id] = e;
+    }
+}
+
+// The fused step: scatter + connect of every cell (strip_task / strip_task16)
+// and, in workgroups [mon_from, gridDim.x), the monitor records.
+template <bool STUBS, bool ENERGY, bool NT, int PF>
+__global__ void __launch_bounds__(256) k_step(StepArgs a, MonitorArgs m, int mon_from)
+{
+    // workgroups from mon_from on take the monitor records of this step
+    // (they read the incident pulses A, which the step only reads, and run
+    // alongside the step's workgroups instead of in a launch of their own)
+    if ((int)blockIdx.x >= mon_from)
+    {
+        monitor_item(m, ((int)blockIdx.x - mon_from) * (int)blockDim.x + (int)threadIdx.x);
+        return;
+    }
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wpb = blockDim.x / kWave;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const XcdRange gr = xcd_range(blockIdx.x, mon_from, a.groups);
+    double esum = 0.0;
+    for (int g = gr.first; g < gr.end; g += gr.step)
+    {
+        const int task = g * wpb + wave;
+        if (task < a.tasks)
+        {
+            if constexpr (PF == PF_F32)
+                strip_task<STUBS, ENERGY, NT, PF>(a, task, lane, esum);
+            else
+                strip_task16<STUBS, ENERGY, PF>(a, task, lane, esum);
+        }
+    }
+    if (ENERGY)
+    {
+        for (int off = kWave / 2; off > 0; off >>= 1)
+            esum += __shfl_down(esum, off, kWave);
+        if (lane == 0)
+            a.epart[blockIdx.x * wpb + wave] = esum;
     }
 }
 
