@@ -60,13 +60,17 @@ class TlmServer:
 
     def __init__(self, device=0, block_ticks=32, hbm_reserve=4 << 30, max_members=1024,
                  tuning=None, autotune_ticks=0, keep_results=True, progress_s=0.0,
-                 max_cells=0):
+                 max_cells=0, retune_s=60.0):
         self.device = device
         self.block_ticks = int(block_ticks)
         self.hbm_reserve = int(hbm_reserve)
         self.max_members = int(max_members)
         self.tuning = tuning
         self.autotune_ticks = int(autotune_ticks)
+        # the launch shape is tuned once the batch holds a representative
+        # load (half the cells in flight, or 8 members without a cell cap)
+        # and again every retune_s seconds, as the mix of members changes
+        self.retune_s = float(retune_s)
         self.keep_results = keep_results
         self.progress_s = float(progress_s)  # > 0: a progress line on stderr this often
         # > 0: admit while the cells in flight stay below this (first come,
@@ -178,9 +182,15 @@ class TlmServer:
                 live[dev] = (r, sc, adm)
             pending.clear()
             if live:
-                if self.autotune_ticks and not tuned and len(live) >= 2:
-                    batch.autotune(self.autotune_ticks)
-                    tuned = True
+                if self.autotune_ticks:
+                    cells = sum(v[1].ncell for v in live.values())
+                    loaded = cells >= 0.5 * self.max_cells if self.max_cells > 0 \
+                        else len(live) >= 8
+                    due = tuned is False or (self.retune_s > 0 and
+                                             time.perf_counter() - tuned >= self.retune_s)
+                    if loaded and due:
+                        batch.autotune(self.autotune_ticks)
+                        tuned = time.perf_counter()
                 batch.run(self.block_ticks)          # queued; the host goes on below
                 ticks += self.block_ticks
                 member_ticks += self.block_ticks * len(live)
