@@ -398,6 +398,15 @@ def cpu_baseline(scene, budget_s=15.0):
     from oracle.oracle_tlm import load_library
     threads = load_library().ot_default_threads()
     o.close()
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {
         "value": scene.ncell * n / dt / 1e9,
         "unit": "Gcell-updates/s",
@@ -405,7 +414,8 @@ def cpu_baseline(scene, budget_s=15.0):
         "kind": "port",
         "sample": f"{scene.name}: steps 2..{n + 1} ({n} steps, {dt:.1f}s) of the same scene with "
                   f"all monitors, CPU restatement of TlmSolver's step with its ThreadPool rule "
-                  f"min(16, hw-1) = {threads} threads; host {os.cpu_count()} logical CPUs",
+                  f"min(16, hw-1) = {threads} threads; host {os.cpu_count()} logical CPUs "
+                  f"({model})",
     }
 
 
