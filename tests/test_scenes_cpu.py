@@ -1,0 +1,58 @@
+"""Scene builders on the CPU: a scene built as device set-up operations is
+the same grid as the host-map scene (its operations replayed by the
+oracle's voxelisers and feed carve), the served stream is reproducible, and
+its summary arithmetic is right."""
+import numpy as np
+import pytest
+
+from emsolver_amd import scenes, serve
+from tests import helpers
+
+
+@pytest.mark.parametrize("dims,frac,boxes,seed", [((36, 30, 44), 0.5, 5, 9),
+                                                    ((32, 41, 57), 0.3, 8, 2),
+                                                    ((64, 33, 38), 0.7, 0, 4)])
+def test_dipole_ops_equal_host_dipole(dims, frac, boxes, seed):
+    kw = dict(length_frac=frac, steps=100, background_boxes=boxes, seed=seed)
+    d = scenes.dipole_scene(*dims, device_setup=True, **kw)
+    h = scenes.dipole_scene(*dims, **kw)
+    r = helpers.oracle_build(d)
+    assert np.array_equal(r.mat, h.mat)
+    assert r.ports == [(list(h.ports[0][0]), 2, 1.0)]
+    assert np.array_equal(d.epsr, h.epsr) and np.array_equal(d.sigma, h.sigma)
+    assert d.has_diel == h.has_diel == (boxes > 0)
+
+
+def test_request_scene_host_and_ops_agree():
+    for dims, st, pulse, sd in (((40, 35, 50), 300, "f16", 3), ((33, 60, 41), 500, "f32", 8)):
+        a = serve.request_scene(dims, 0.5, st, pulse, sd)
+        b = serve.request_scene(dims, 0.5, st, pulse, sd, device_setup=True)
+        assert np.array_equal(a.probes, b.probes)
+        rb = helpers.oracle_build(b)
+        assert np.array_equal(rb.mat, a.mat) and rb.ports == [(list(a.ports[0][0]), 2, 1.0)]
+        assert a.pulse == b.pulse == pulse and a.total_steps == b.total_steps == st
+        # probes keep off the dipole column, the only PEC cells
+        assert (a.mat[a.probes] != scenes.MAT_PEC).all()
+
+
+def test_served_stream_is_reproducible():
+    a = serve.served_stream(count=50, seed=5)
+    b = serve.served_stream(count=50, seed=5)
+    assert [(r.rid, r.arrival, r.tol) for r in a] == [(r.rid, r.arrival, r.tol) for r in b]
+    t = np.array([r.arrival for r in a])
+    # 50 Poisson arrivals at 50 per second: about one second
+    assert (np.diff(t) > 0).all() and 0.5 < t[-1] < 2.0
+    assert {r.tol for r in a} == {1e-3, 1e-5}
+    sa, sb = a[7].build(), b[7].build()
+    assert sa.name == sb.name and np.array_equal(sa.mat, sb.mat)
+    assert 32 <= min(sa.nx, sa.ny, sa.nz) and max(sa.nx, sa.ny, sa.nz) <= 384
+
+
+def test_summarize():
+    done = [serve.Completion(i, float(i), float(i), float(i) + 1.0 + i, 1000, 10, "f32")
+            for i in range(5)]
+    s = serve.summarize(done)
+    lat = np.array([1.0 + i for i in range(5)])
+    assert s["requests"] == 5 and s["p50_latency_s"] == np.percentile(lat, 50)
+    assert s["cell_updates"] == 5 * 1000 * 10
+    assert abs(s["gcell_s"] - 5e4 / (9.0 - 0.0) / 1e9) < 1e-15
