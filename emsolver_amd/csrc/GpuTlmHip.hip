@@ -2,10 +2,12 @@
 //
 // Replaces the OpenGL set-up and per-step dispatch of GpuTlm::run
 // (GpuTlm.cpp:532-884 in the reference) with HIP: one stream per context,
-// device-resident per-step tables, three launches per step (inject,
-// monitors, fused scatter/connect) and no host synchronisation inside a run.
+// device-resident per-step tables, two launches per step (inject, then the
+// fused scatter/connect whose extra workgroups take the monitor records) and
+// no host synchronisation inside a run.
 #include "tlm_context.h"
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -70,9 +72,13 @@ size_t energy_parts(const tlm_ctx *c)
 }
 
 // The fused step launch; nmon > 0 appends the workgroups that take the
-// step's monitor records.
+// step's monitor records. With events e0 / e1 the launch carries them as its
+// own start / end timestamps (hipExtLaunchKernel): a marker recorded on the
+// stream between two kernels would add a gap of ~5 us before and after the
+// step, which hipEventRecord on either side of it measured.
 template <bool S, bool E, int PF>
-void launch_fused(tlm_ctx *c, const tlm::StepArgs &a, const tlm::MonitorArgs &m, int nmon)
+void launch_fused(tlm_ctx *c, const tlm::StepArgs &a, const tlm::MonitorArgs &m, int nmon,
+                  hipEvent_t e0, hipEvent_t e1)
 {
     const int bt = c->wpb * tlm::kWave;
     const int mon_blocks = nmon > 0 ? (nmon + bt - 1) / bt : 0;
@@ -80,23 +86,32 @@ void launch_fused(tlm_ctx *c, const tlm::StepArgs &a, const tlm::MonitorArgs &m,
     // automatic: non-temporal pulse stores when the grid has stub cells
     // (faster there by 1-3 %; slower by 3-4 % in plain air)
     const bool nt = c->tune.nontemporal == 1 || (c->tune.nontemporal == 2 && c->stubs);
+    auto go = [&](auto kern) {
+        if (e0)
+            hipExtLaunchKernelGGL(kern, grid, block, 0, c->stream, e0, e1, 0, a, m, c->blocks);
+        else
+            hipLaunchKernelGGL(kern, grid, block, 0, c->stream, a, m, c->blocks);
+    };
     if (nt && PF == tlm::PF_F32)
-        hipLaunchKernelGGL((tlm::k_step<S, E, true, PF>), grid, block, 0, c->stream, a, m, c->blocks);
+        go(tlm::k_step<S, E, true, PF>);
     else
-        hipLaunchKernelGGL((tlm::k_step<S, E, false, PF>), grid, block, 0, c->stream, a, m, c->blocks);
+        go(tlm::k_step<S, E, false, PF>);
 }
 
 template <int PF>
-void launch_pf(tlm_ctx *c, const tlm::StepArgs &a, bool energy, const tlm::MonitorArgs &m, int nmon)
+void launch_pf(tlm_ctx *c, const tlm::StepArgs &a, bool energy, const tlm::MonitorArgs &m, int nmon,
+               hipEvent_t e0, hipEvent_t e1)
 {
     if (c->stubs)
-        energy ? launch_fused<true, true, PF>(c, a, m, nmon) : launch_fused<true, false, PF>(c, a, m, nmon);
+        energy ? launch_fused<true, true, PF>(c, a, m, nmon, e0, e1)
+               : launch_fused<true, false, PF>(c, a, m, nmon, e0, e1);
     else
-        energy ? launch_fused<false, true, PF>(c, a, m, nmon) : launch_fused<false, false, PF>(c, a, m, nmon);
+        energy ? launch_fused<false, true, PF>(c, a, m, nmon, e0, e1)
+               : launch_fused<false, false, PF>(c, a, m, nmon, e0, e1);
 }
 
 
-int launch_step(tlm_ctx *c, int n, const tlm::MonitorArgs &m, int nmon)
+int launch_step(tlm_ctx *c, int n, const tlm::MonitorArgs &m, int nmon, hipEvent_t e0, hipEvent_t e1)
 {
     const bool energy = c->every > 0 && (n % c->every) == 0;
     tlm::StepArgs a;
@@ -133,11 +148,11 @@ int launch_step(tlm_ctx *c, int n, const tlm::MonitorArgs &m, int nmon)
     a.groups = c->groups;
     a.rho = c->rho;
     if (c->pf == TLM_PULSE_F16)
-        launch_pf<tlm::PF_F16>(c, a, energy, m, nmon);
+        launch_pf<tlm::PF_F16>(c, a, energy, m, nmon, e0, e1);
     else if (c->pf == TLM_PULSE_BF16)
-        launch_pf<tlm::PF_BF16>(c, a, energy, m, nmon);
+        launch_pf<tlm::PF_BF16>(c, a, energy, m, nmon, e0, e1);
     else
-        launch_pf<tlm::PF_F32>(c, a, energy, m, nmon);
+        launch_pf<tlm::PF_F32>(c, a, energy, m, nmon, e0, e1);
     if (c->pf != TLM_PULSE_F32)
         hipLaunchKernelGGL(tlm::k_slab_max, dim3(c->nz), dim3(256), 0, c->stream, c->mpart,
                            c->strips, c->mreg);
@@ -203,7 +218,8 @@ int queue_steps(tlm_ctx *c, int step0, int nsteps, bool time_kernel, float *ms_k
     const int nmon0 = c->nf + c->nh + c->np + c->npr;
     if (time_kernel)
     {
-        // events bracket every fused-kernel launch; no host wait inside the loop
+        // every fused-kernel launch carries a start and an end event; no host
+        // wait inside the loop
         while ((int)c->kev.size() < 2 * nsteps)
         {
             hipEvent_t ev;
@@ -283,10 +299,9 @@ int queue_steps(tlm_ctx *c, int step0, int nsteps, bool time_kernel, float *ms_k
         }
         // the monitor records of step n run in the step launch's extra workgroups
         if (time_kernel)
-            HIPCHK(c, hipEventRecord(c->kev[2 * (n - step0)], c->stream));
-        launch_step(c, n, m, nmon);
-        if (time_kernel)
-            HIPCHK(c, hipEventRecord(c->kev[2 * (n - step0) + 1], c->stream));
+            launch_step(c, n, m, nmon, c->kev[2 * (n - step0)], c->kev[2 * (n - step0) + 1]);
+        else
+            launch_step(c, n, m, nmon, nullptr, nullptr);
         if (c->frame_every > 0 && (n % c->frame_every) == 0 &&
             (int)c->frame_steps.size() < c->frame_max)
         {
