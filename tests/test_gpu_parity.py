@@ -33,9 +33,11 @@ def _random_scene(nx, ny, nz, seed, rho=0.0, pec_frac=0.05, diel=True, steps=40,
     sc = scenes.Scene(nx, ny, nz, mat, epsr, sigma, rho=rho, total_steps=steps, settle_steps=5,
                       name=f"random_{nx}x{ny}x{nz}_s{seed}")
     air = np.flatnonzero(mat == 0)
+    if air.size == 0:  # a grid of a few cells: put the sources in dielectric cells
+        air = np.flatnonzero(mat != scenes.MAT_PEC)
     sc.sources = [(int(air[len(air) // 2]), 2, 1.0), (int(air[len(air) // 3]), 0, -0.7),
                   (int(air[len(air) // 2]), 1, 0.25)]
-    sc.probes = np.array(rng.choice(n, 5, replace=False), np.int64)
+    sc.probes = np.array(rng.choice(n, min(5, n), replace=False), np.int64)
     sc.dft_freqs = np.array([1e8, 5e8, 1e9])
     if random_state:
         V0 = rng.standard_normal(12 * n).astype(np.float32)
@@ -65,6 +67,20 @@ def test_device_present():
 @pytest.mark.parametrize("rho", [0.0, -1.0, 0.37])
 def test_random_materials_bitexact(shape, rho):
     sc = _random_scene(*shape, seed=sum(shape), rho=rho, steps=24)
+    _check(sc)
+
+
+# Grids one or two cells thick: every cell of a 1-wide axis touches both
+# outer faces of it, rows shorter than a wave, strips holding many rows.
+THIN = [(1, 1, 1), (1, 4, 5), (5, 1, 4), (4, 5, 1), (2, 2, 2), (65, 1, 1), (1, 1, 70),
+        (1, 70, 1), (3, 1, 66)]
+
+
+@pytest.mark.parametrize("shape", THIN, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("rho", [-1.0, 0.37])
+def test_thin_grids_bitexact(shape, rho):
+    n = shape[0] * shape[1] * shape[2]
+    sc = _random_scene(*shape, seed=n + 11, rho=rho, pec_frac=0.0 if n < 8 else 0.1, steps=30)
     _check(sc)
 
 

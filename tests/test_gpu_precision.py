@@ -89,6 +89,15 @@ def test_random_materials_bitexact(shape, rho):
     _check16(dataclasses.replace(sc, pulse="bf16", fp32_halo=-1))
 
 
+@pytest.mark.parametrize("shape", [(1, 1, 1), (4, 5, 1), (1, 1, 70), (65, 1, 3)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_thin_grids_bitexact(shape):
+    n = shape[0] * shape[1] * shape[2]
+    sc = _random_scene(*shape, seed=n + 5, rho=0.37, pec_frac=0.0 if n < 8 else 0.1, steps=30)
+    _check16(dataclasses.replace(sc, pulse="f16", fp32_halo=1))
+    _check16(dataclasses.replace(sc, pulse="bf16", fp32_halo=-1))
+
+
 def test_dipole_port_bitexact():
     sc = scenes.dipole_scene(24, 22, 40, length_frac=0.5, steps=150, background_boxes=3, seed=2)
     sc.probes = np.array([sc.cell(3, 4, 5), sc.cell(20, 11, 30), sc.cell(12, 11, 20)], np.int64)
