@@ -61,6 +61,23 @@ def test_batch_bitexact(tuning):
         _compare(g, _oracle(sc), sc)
 
 
+def test_batch_thin_members():
+    """Members one or two cells thick next to an ordinary one."""
+    scs = []
+    for i, shape in enumerate([(1, 1, 1), (4, 5, 1), (1, 1, 70), (65, 1, 3), (2, 2, 2)]):
+        n = shape[0] * shape[1] * shape[2]
+        sc = _random_scene(*shape, seed=n + 13, rho=0.37, pec_frac=0.0 if n < 8 else 0.1,
+                           steps=20 + 3 * i)
+        sc.faces = False
+        scs.append(sc)
+    scs.append(dataclasses.replace(scs[3], pulse="f16", fp32_halo=1, name="thin_f16"))
+    scs.append(scenes.batch_dipoles(count=1, lo=12, hi=20, steps=(30, 31), seed=3, nprobes=2)[0])
+    print(f"[batch] {len(scs)} thin members", flush=True)
+    res = tlm.run_scenes_batched(scs, block=9)
+    for g, sc in zip(res, scs):
+        _compare(g, _oracle(sc), sc)
+
+
 def test_batch_join_and_leave():
     """Continuous batching: members join at different ticks, finished ones
     leave, new ones take their place; every result is the oracle's."""
