@@ -81,10 +81,18 @@ def read_traffic(workload):
     return float(m.group(1)) if m else None
 
 
+# Every KERNEL_EVERY-th step launch of a timed region carries a start and an
+# end HIP event for the kernel average. Events on a launch cost host time (~4
+# us per launch at 256^3), so a sample perturbs the timed steps less than
+# events on all of them; 7 is prime to the 16-step energy period, so the
+# energy steps' launches enter the sample at their own rate.
+KERNEL_EVERY = 7
+
+
 def gpu_run(scene, warmup, steps, tag, plain=False):
-    """Time `steps` steps after `warmup`, each step launch carrying HIP events
-    for the kernel average. plain=True times `steps` more steps without the
-    events (the run's rate when nothing measures single launches)."""
+    """Time `steps` steps after `warmup`, every KERNEL_EVERY-th step launch
+    carrying HIP events for the kernel average. plain=True times `steps`
+    more steps with no events at all."""
     from emsolver_amd import tlm
     sc = scene.with_steps(warmup + steps * (2 if plain else 1))
     t_setup = time.perf_counter()
@@ -94,7 +102,7 @@ def gpu_run(scene, warmup, steps, tag, plain=False):
         dev.run(0, warmup)
     dev.sync()
     t0 = time.perf_counter()
-    _, ms_kernel = dev.time_steps(warmup, steps, kernel=True)
+    _, ms_kernel = dev.time_steps(warmup, steps, kernel_every=KERNEL_EVERY)
     dev.sync()
     wall = time.perf_counter() - t0
     cells = sc.ncell
@@ -102,11 +110,12 @@ def gpu_run(scene, warmup, steps, tag, plain=False):
         "wall_s": wall,
         "gcells": cells * steps / wall / 1e9,
         "ms_per_step": wall * 1e3 / steps,
-        "kernel_ms_avg": ms_kernel / steps,
+        "kernel_ms_avg": ms_kernel,
+        "kernel_launches_timed": len(range(0, steps, KERNEL_EVERY)),
     }
     if plain:
         t1 = time.perf_counter()
-        dev.time_steps(warmup + steps, steps, kernel=False)
+        dev.time_steps(warmup + steps, steps, kernel_every=0)
         dev.sync()
         out["plain_gcells"] = cells * steps / (time.perf_counter() - t1) / 1e9
     b = alg_bytes_per_cell(sc)
@@ -393,6 +402,58 @@ def ntff_run(n=256, offset=4, n_theta=37, n_phi=73, cpu_every=32):
     return out
 
 
+def free_space_64_run(with_cpu=True):
+    """The reference's CPU-runnable case: 64^3 free space, matched
+    boundaries, centre Ez source, Ez probe 10 cells off-centre, 1000 steps.
+    Its ~25 MB of state stays in the Infinity Cache, so its bandwidth is
+    reported but it is no HBM roofline case. With with_cpu the oracle runs
+    the same 1000 steps (the CPU baseline of this case) and the GPU probe
+    series is compared with it bit for bit."""
+    from emsolver_amd import scenes, tlm
+    sc = scenes.free_space_cube(n=64, steps=1000, probe_offset=10)
+    dev = tlm.open_scene(sc)
+    dev.sync()
+    t0 = time.perf_counter()
+    dev.time_steps(0, sc.total_steps, kernel_every=0)
+    dev.sync()
+    wall = time.perf_counter() - t0
+    g = tlm.collect(dev, sc, sc.total_steps, state=False)
+    dev.close()
+    # kernel average from a second run of the same scene with events on every
+    # launch (they cost ~20 us of host time per launch at this size)
+    with tlm.open_scene(sc) as dev:
+        _, ker = dev.time_steps(0, sc.total_steps, kernel_every=1)
+    b = alg_bytes_per_cell(sc)
+    out = {
+        "grid": [64, 64, 64],
+        "steps": sc.total_steps,
+        "gcell_s": round(sc.ncell * sc.total_steps / wall / 1e9, 3),
+        "kernel_us_avg": round(ker * 1e3, 2),
+        "achieved_gbs": round(sc.ncell * b / (ker * 1e-3) / 1e9, 1),
+        "note": "state (25 MB) resident in the Infinity Cache: not an HBM roofline case; "
+                "gcell_s from 1000 steps with no launch events, kernel_us_avg from a second "
+                "run with events on every launch",
+    }
+    log(f"[bench] free_space_64: {out['gcell_s']:.2f} Gcell/s wall, kernel {out['kernel_us_avg']:.1f} us")
+    if with_cpu:
+        from tests import helpers
+        o = helpers.oracle_open(sc)
+        t0 = time.perf_counter()
+        o.run_steps(sc.total_steps)
+        dt = time.perf_counter() - t0
+        o.finalize()
+        r = helpers.oracle_collect(o, sc)
+        o.close()
+        from oracle.oracle_tlm import load_library
+        out["cpu_gcell_s"] = round(sc.ncell * sc.total_steps / dt / 1e9, 4)
+        out["cpu_threads"] = load_library().ot_default_threads()
+        out["cpu_sample"] = "all 1000 steps, CPU restatement of TlmSolver's step"
+        out["probe_series_bitexact"] = bool(np.array_equal(g.probes, r.probes))
+        log(f"[bench] free_space_64 CPU {out['cpu_gcell_s']:.3f} Gcell/s, probe series "
+            f"bit-exact: {out['probe_series_bitexact']}")
+    return out
+
+
 def cpu_baseline(scene, budget_s=15.0):
     from tests import helpers
     o = helpers.oracle_open(scene.with_steps(10**6))
@@ -440,7 +501,7 @@ def main():
                     help="timed steps of each extra 512^3 line")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched and served lines")
     ap.add_argument("--lines", default="",
-                    help="comma list of extra lines to run (diel_boxes_512_f32, diel_boxes_512_f16, "
+                    help="comma list of extra lines to run (free_space_64, diel_boxes_512_f32, diel_boxes_512_f16, "
                          "diel_boxes_512_bf16, free_space_512_f32, scene_setup_256, "
                          "scene_setup_512, ntff_256, batched_256, served); default all")
     ap.add_argument("--no-graph", action="store_true",
@@ -494,8 +555,9 @@ def main():
     extra = {"uninstrumented": {
         "gcell_s": round(main_run["plain_gcells"], 3),
         "steps": args.steps,
-        "note": "the next steps of the same run with no HIP events on the launches; value "
-                "carries a start and an end event on every step launch for the kernel average"}}
+        "note": f"the next steps of the same run with no HIP events on the launches; in the "
+                f"timed steps every {KERNEL_EVERY}th step launch carries a start and an end "
+                f"event for the kernel average"}}
     lib = tlm._native.load()
     import ctypes as C
     gbps, msc = C.c_float(0), C.c_float(0)
@@ -539,6 +601,8 @@ def main():
         log(f"[bench] free_space_512: {r['gcells']:.2f} Gcell/s, kernel {r['kernel_ms_avg']:.3f} ms, "
             f"{r['achieved_gbs']:.0f} GB/s = {r['achieved_gbs'] / HBM_PEAK_GBS:.3f} of peak")
 
+    if not args.no_extra and want("free_space_64"):
+        extra["free_space_64"] = free_space_64_run(with_cpu=not args.no_cpu)
     for sn in (256, 512):
         if not args.no_extra and want(f"scene_setup_{sn}"):
             extra[f"scene_setup_{sn}"] = setup_run(sn)
@@ -588,6 +652,7 @@ def main():
             "traffic": read_traffic(workload),
             "kernel": "tlm::k_step (fused scatter+connect)",
             "kernel_ms_avg": round(main_run["kernel_ms_avg"], 5),
+            "kernel_launches_timed": main_run["kernel_launches_timed"],
             "alg_bytes_per_launch": int(round(scene.ncell * main_run["alg_bytes_per_cell"])),
         },
         "cpu_baseline": cpu,

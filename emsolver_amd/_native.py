@@ -72,7 +72,7 @@ SIGNATURES = {
     "tlm_read_js": (C.c_int, [P, fp, fp]),
     "tlm_read_huygens": (C.c_int, [P, fp]),
     "tlm_read_energy": (C.c_int, [P, i32p, dp, C.c_int]),
-    "tlm_time_steps": (C.c_int, [P, C.c_int, C.c_int, fp, fp]),
+    "tlm_time_steps": (C.c_int, [P, C.c_int, C.c_int, C.c_int, fp, fp, C.POINTER(C.c_int)]),
     "tlm_launch_shape": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                    C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "tlm_set_tuning": (C.c_int, [P, C.POINTER(Tuning)]),

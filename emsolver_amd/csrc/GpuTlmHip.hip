@@ -191,7 +191,10 @@ tlm::PulseView pulse_view(const tlm_ctx *c, int n)
 }
 
 
-int queue_steps(tlm_ctx *c, int step0, int nsteps, bool time_kernel, float *ms_kernel)
+// kernel_every > 0: the step launches i = 0, k, 2k, ... of the block carry
+// timing events; their summed duration goes to *ms_kernel, their count to
+// *nkernel.
+int queue_steps(tlm_ctx *c, int step0, int nsteps, int kernel_every, float *ms_kernel, int *nkernel)
 {
     if (c->batch)
         return fail(c, TLM_ERR_STATE, "context is a batch member (tlm_batch_remove it first)");
@@ -216,11 +219,13 @@ int queue_steps(tlm_ctx *c, int step0, int nsteps, bool time_kernel, float *ms_k
             return fail(c, TLM_ERR_STATE, "16-bit pulse storage runs contiguous step blocks");
     }
     const int nmon0 = c->nf + c->nh + c->np + c->npr;
+    const bool time_kernel = kernel_every > 0;
+    const int ntimed = time_kernel ? (nsteps + kernel_every - 1) / kernel_every : 0;
     if (time_kernel)
     {
-        // every fused-kernel launch carries a start and an end event; no host
-        // wait inside the loop
-        while ((int)c->kev.size() < 2 * nsteps)
+        // the sampled launches carry a start and an end event; no host wait
+        // inside the loop
+        while ((int)c->kev.size() < 2 * ntimed)
         {
             hipEvent_t ev;
             HIPCHK(c, hipEventCreate(&ev));
@@ -298,8 +303,9 @@ int queue_steps(tlm_ctx *c, int step0, int nsteps, bool time_kernel, float *ms_k
                 c->plane_step = n;
         }
         // the monitor records of step n run in the step launch's extra workgroups
-        if (time_kernel)
-            launch_step(c, n, m, nmon, c->kev[2 * (n - step0)], c->kev[2 * (n - step0) + 1]);
+        const int ti = n - step0;
+        if (time_kernel && ti % kernel_every == 0)
+            launch_step(c, n, m, nmon, c->kev[2 * (ti / kernel_every)], c->kev[2 * (ti / kernel_every) + 1]);
         else
             launch_step(c, n, m, nmon, nullptr, nullptr);
         if (c->frame_every > 0 && (n % c->frame_every) == 0 &&
@@ -353,14 +359,16 @@ int queue_steps(tlm_ctx *c, int step0, int nsteps, bool time_kernel, float *ms_k
     if (time_kernel && ms_kernel)
     {
         float kern_ms = 0.f;
-        HIPCHK(c, hipEventSynchronize(c->kev[2 * nsteps - 1]));
-        for (int s = 0; s < nsteps; ++s)
+        HIPCHK(c, hipEventSynchronize(c->kev[2 * ntimed - 1]));
+        for (int s = 0; s < ntimed; ++s)
         {
             float ms = 0.f;
             HIPCHK(c, hipEventElapsedTime(&ms, c->kev[2 * s], c->kev[2 * s + 1]));
             kern_ms += ms;
         }
         *ms_kernel = kern_ms;
+        if (nkernel)
+            *nkernel = ntimed;
     }
     HIPCHK(c, hipGetLastError());
     return TLM_OK;
@@ -1136,7 +1144,7 @@ int tlm_run(tlm_ctx *c, int step0, int nsteps)
 {
     if (!c)
         return TLM_ERR_ARG;
-    return queue_steps(c, step0, nsteps, false, nullptr);
+    return queue_steps(c, step0, nsteps, 0, nullptr, nullptr);
 }
 
 int tlm_sync(tlm_ctx *c)
@@ -1246,13 +1254,20 @@ int tlm_device_encode(int device, int fmt, const float *in, float inv_scale, uin
     return rc;
 }
 
-int tlm_time_steps(tlm_ctx *c, int step0, int nsteps, float *ms_total, float *ms_kernel)
+int tlm_time_steps(tlm_ctx *c, int step0, int nsteps, int kernel_every, float *ms_total,
+                   float *ms_kernel, int *nkernel)
 {
     if (!c)
         return TLM_ERR_ARG;
+    if (kernel_every < 0)
+        return fail(c, TLM_ERR_ARG, "tlm_time_steps: kernel_every < 0");
+    if (ms_kernel)
+        *ms_kernel = 0.f;
+    if (nkernel)
+        *nkernel = 0;
     HIPCHK(c, hipSetDevice(c->dev));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    int rc = queue_steps(c, step0, nsteps, ms_kernel != nullptr, ms_kernel);
+    int rc = queue_steps(c, step0, nsteps, ms_kernel ? kernel_every : 0, ms_kernel, nkernel);
     if (rc)
         return rc;
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));

@@ -365,12 +365,18 @@ class TlmDevice:
     def sync(self):
         self._chk(self.lib.tlm_sync(self.h), "tlm_sync")
 
-    def time_steps(self, step0, nsteps, kernel=True):
+    def time_steps(self, step0, nsteps, kernel_every=1):
+        """Run and time steps [step0, step0+nsteps): (ms of the whole block,
+        average ms of one fused step launch over the launches of every
+        kernel_every-th step, or None with kernel_every=0)."""
         tot = C.c_float(0)
         ker = C.c_float(0)
-        self._chk(self.lib.tlm_time_steps(self.h, int(step0), int(nsteps), C.byref(tot),
-                                          C.byref(ker) if kernel else None), "tlm_time_steps")
-        return tot.value, (ker.value if kernel else None)
+        nk = C.c_int(0)
+        k = int(kernel_every)
+        self._chk(self.lib.tlm_time_steps(self.h, int(step0), int(nsteps), k, C.byref(tot),
+                                          C.byref(ker) if k > 0 else None, C.byref(nk)),
+                  "tlm_time_steps")
+        return tot.value, (ker.value / nk.value if k > 0 and nk.value > 0 else None)
 
     def set_tuning(self, kernel=0, planes=1, block_waves=0, nontemporal=2, grid_blocks=0):
         t = _native.Tuning(int(kernel), int(planes), int(block_waves), int(nontemporal),

@@ -196,9 +196,15 @@ int tlm_read_plane(tlm_ctx *ctx, float *out, int *n1, int *n2, int *step);
 int tlm_read_frames(tlm_ctx *ctx, int32_t *steps, float *js, float *planes, int cap);
 
 /* Runs steps [step0, step0+nsteps) and measures them with HIP events on the
- * context stream: *ms_total spans the whole queue of the steps, *ms_kernel
- * is the summed duration of the fused scatter/connect launches alone. */
-int tlm_time_steps(tlm_ctx *ctx, int step0, int nsteps, float *ms_total, float *ms_kernel);
+ * context stream: *ms_total spans the whole queue of the steps. With
+ * kernel_every = k > 0 (and ms_kernel not NULL) the fused scatter/connect
+ * launches of the steps step0, step0+k, step0+2k, ... carry their own start
+ * and end events: *ms_kernel is their summed duration and *nkernel (may be
+ * NULL) their count. Events on a launch cost host time (~4 us per launch at
+ * 256^3, ~20 us at 64^3), so a sample (k > 1) perturbs the run less than
+ * k = 1; k = 0 measures no launch. */
+int tlm_time_steps(tlm_ctx *ctx, int step0, int nsteps, int kernel_every, float *ms_total,
+                   float *ms_kernel, int *nkernel);
 
 /* Device-copy bandwidth: copies `bytes` between two device buffers `iters`
  * times with a plain grid-stride float4 kernel (k_copy) and reports
