@@ -698,6 +698,14 @@ __device__ __forceinline__ float pv_set(const PulseView &v, int p, long long c, 
     return fabsf(y);
 }
 
+// Stores a 16-bit code into a slot of a 16-bit slab.
+__device__ __forceinline__ void pv_set_code(const PulseView &v, int p, long long c, unsigned short h)
+{
+    const long long k = c / v.Pp, f = c - k * v.Pp;
+    char *q = v.base + slab16_base(v.Pp, k, v.h0, v.h1) + (size_t)(p >> 1) * 64u * 4u + blk16((unsigned)f, 4u);
+    reinterpret_cast<unsigned short *>(q)[p & 1] = h;
+}
+
 // Slab exponent for a bound: bound * 2^-E < 2^15, E in [-100, 100]
 // (identical integer steps on the host and in the oracle).
 __host__ __device__ inline int plan_exponent(float bound)
@@ -764,23 +772,70 @@ __device__ __forceinline__ void plan_slab(const PlanArgs &p, int k)
 
 // Adds the sources [s0, s1) of one cell in list order (0.5*amp*sv on the four
 // ports polarised along pol); returns the largest |code| a 16-bit slot took.
+// The cell's 12 slots are loaded together, the additions done in registers
+// and the touched slots stored once, so the thread waits for memory twice,
+// not once per addition. A 16-bit slot is rounded after each addition, the
+// value a store and a new load of the slot would give, so every slot takes
+// the same sequence of values as with one read-modify-write per addition.
 __device__ __forceinline__ float inject_cell(const PulseView &pv, long long c, int s0, int s1,
                                              const int *spol, const float *samp, float sv)
 {
+    const long long k = c / pv.Pp;
+    const bool h16 = pv.pf != PF_F32 && pv.slab[k].y != 0;
+    const float sc = h16 ? pv.sc[k] : 1.0f, isc = h16 ? pv.isc[k] : 1.0f;
+    float v[12];
+#pragma unroll
+    for (int p = 0; p < 12; ++p)
+        v[p] = pv_get(pv, p, c);
+    unsigned short hc[12] = {};
     float over = 0.f;
+    unsigned touched = 0u;
     for (int s = s0; s < s1; ++s)
     {
         const int pol = spol[s];
         const float qv = 0.5f * samp[s] * sv;
-        for (int axis = 0; axis < 3; ++axis)
+#pragma unroll
+        for (int pp = 0; pp < 3; ++pp)
         {
-            if (axis == pol)
+            if (pp != pol)
                 continue;
-            const int p0 = scn::port_of(axis, 0, pol), p1 = scn::port_of(axis, 1, pol);
-            over = fmaxf(over, pv_set(pv, p0, c, pv_get(pv, p0, c) + qv));
-            over = fmaxf(over, pv_set(pv, p1, c, pv_get(pv, p1, c) + qv));
+#pragma unroll
+            for (int axis = 0; axis < 3; ++axis)
+            {
+                if (axis == pp)
+                    continue;
+#pragma unroll
+                for (int side = 0; side < 2; ++side)
+                {
+                    const int p = scn::port_of(axis, side, pp);
+                    float x = v[p] + qv;
+                    if (h16)
+                    {
+                        const float y = x * isc;
+                        const unsigned short h =
+                            pv.pf == PF_F16 ? enc16<PF_F16>(y) : enc16<PF_BF16>(y);
+                        over = fmaxf(over, fabsf(y));
+                        hc[p] = h;
+                        x = (pv.pf == PF_F16 ? dec16<PF_F16>(h) : dec16<PF_BF16>(h)) * sc;
+                    }
+                    v[p] = x;
+                    touched |= 1u << p;
+                }
+            }
         }
     }
+    // a 16-bit slot takes the code of its last addition as it is (its
+    // decoded value times 2^-E need not give the code back where the decode
+    // leaves the float range)
+#pragma unroll
+    for (int p = 0; p < 12; ++p)
+        if (touched & (1u << p))
+        {
+            if (h16)
+                pv_set_code(pv, p, c, hc[p]);
+            else
+                pv_set(pv, p, c, v[p]);
+        }
     return over;
 }
 
