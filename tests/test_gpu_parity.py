@@ -170,3 +170,21 @@ def test_plane_and_frames_bitexact():
     _check(sc)
     sc.plane_axis, sc.plane_index = 0, 99   # clamped to the last x plane
     _check(sc.with_steps(70))
+
+
+@pytest.mark.parametrize("every", [1, 3])
+def test_timed_steps_equal_untimed(every):
+    """tlm_time_steps (events on every `every`-th step launch) runs the same
+    steps as tlm_run: the oracle's bits, and a kernel average that is > 0."""
+    sc = _random_scene(40, 9, 11, seed=17, rho=0.37, steps=23)
+    print(f"[parity] timed steps, events every {every}", flush=True)
+    with tlm.open_scene(sc) as dev:
+        tot, ker = dev.time_steps(0, 10, kernel_every=every)
+        assert tot > 0 and ker is not None and 0 < ker <= tot
+        _, none = dev.time_steps(10, 13, kernel_every=0)
+        assert none is None
+        with pytest.raises(tlm.TlmError, match="kernel_every"):
+            dev.time_steps(23, 0, kernel_every=-1)
+        dev.sync()
+        g = tlm.collect(dev, sc, sc.total_steps)
+    helpers.compare_results(g, helpers.oracle_run(sc), sc)
