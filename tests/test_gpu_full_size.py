@@ -5,6 +5,8 @@
   64-box scene (134 M cells; FP32 and FP16 storage) against the oracle bit
   for bit over their first steps, at the sizes where offsets, launch grids
   and the XCD block ranges take their full-scale values.
+* The batched configuration's 256 grids and requests of the served workload
+  at its full range of sizes: the same bits in the mix as alone.
 * Over a full 3000-step run at 256^3, properties that hold at any size: a
   lossless closed box (rho = -1, sigma = 0, PEC ground plane) keeps its
   weighted energy sum(V^2) + sum(yh * Vs^2) (the quantity the stub node's
@@ -17,7 +19,8 @@ import dataclasses
 import numpy as np
 import pytest
 
-from emsolver_amd import scenes, tlm
+from emsolver_amd import scenes, serve, tlm
+from tests import helpers
 from tests.test_gpu_parity import _check
 from tests.test_gpu_precision import _check16
 
@@ -97,3 +100,44 @@ def test_probe_dft_equals_dft_of_series_3000_steps():
     err = max(np.abs(gre - re).max() / scale.max(), np.abs(gim - im).max() / scale.max())
     print(f"[full] running DFT against the series' DFT: {err:.2e} of sum|x|", flush=True)
     assert err < 1e-12
+
+
+def test_batched_256_grids_equal_alone():
+    """The batched configuration's 256 dipole grids (each axis log-uniform in
+    [32, 128], dielectric background) in one batch with their steps cut to
+    U[60, 160]: every grid's probes, ports and final state equal the same grid
+    run alone, and four of them equal the oracle."""
+    scs = scenes.batch_dipoles(count=256, steps=(60, 160), seed=3)
+    print(f"[full] {len(scs)} grids, {sum(sc.ncell for sc in scs)} cells in one batch", flush=True)
+    res = tlm.run_scenes_batched(scs)
+    for i, (g, sc) in enumerate(zip(res, scs)):
+        alone = tlm.run_scene(sc)
+        helpers.assert_bitexact(f"grid {i} V (batch vs alone)", g.V, alone.V)
+        helpers.assert_bitexact(f"grid {i} probes (batch vs alone)", g.probes, alone.probes)
+        helpers.assert_bitexact(f"grid {i} ports (batch vs alone)", g.ports, alone.ports)
+    for i in (0, 85, 170, 255):
+        o = helpers.oracle_run(scs[i])
+        assert np.abs(o.ports).max() > 0
+        helpers.assert_bitexact(f"grid {i} V (batch vs oracle)", res[i].V, o.V)
+        helpers.assert_bitexact(f"grid {i} ports (batch vs oracle)", res[i].ports, o.ports)
+
+
+def test_served_stream_full_sizes_equal_alone():
+    """Twelve requests of the served workload (each axis log-uniform in
+    [32, 384], FP16 or FP32 storage, steps cut to U[40, 120]) all arriving at
+    once: served together, each one's probes and ports equal the same request
+    run alone."""
+    reqs = serve.served_stream(count=12, rate=1e9, steps=(40, 120), seed=5)
+    for r in reqs:
+        r.arrival = 0.0
+    srv = serve.TlmServer(block_ticks=16)
+    done = srv.serve(reqs, realtime=False)
+    print(f"[full] served {len(done)} requests, largest {max(r.build().ncell for r in reqs)} "
+          f"cells, mean members {srv.stats['mean_members']:.2f}", flush=True)
+    assert sorted(c.rid for c in done) == list(range(len(reqs)))
+    assert srv.stats["mean_members"] > 2
+    for c in done:
+        alone = tlm.run_scene(reqs[c.rid].build())
+        assert np.abs(alone.ports).max() > 0
+        helpers.assert_bitexact(f"req{c.rid} probes (mix vs alone)", c.probes, alone.probes)
+        helpers.assert_bitexact(f"req{c.rid} ports (mix vs alone)", c.ports, alone.ports)
