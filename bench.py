@@ -14,8 +14,8 @@ Prints ONE JSON line. `value` is Gcell-updates/s of the timed K steps
 (wall clock between two stream synchronisations). `roofline` prices the
 fused scatter/connect kernel: algorithmic bytes per launch
 (cells * (97 + 24 * f_diel) B for FP32 pulses) over its average launch
-time, measured with HIP events on the context stream inside the timed
-region, against the 8 TB/s HBM peak. `cpu_baseline` times the CPU
+time, measured with HIP events carried by every 7th step launch of the
+timed region on the context stream, against the 8 TB/s HBM peak. `cpu_baseline` times the CPU
 restatement of the reference algorithm (oracle/, the reference's threading
 rule min(16, hw-1)) on a bounded sample of the same workload.
 """
