@@ -516,6 +516,8 @@ def main():
                          "set-up operations voxelised on the device)")
     ap.add_argument("--serve-count", type=int, default=200,
                     help="requests of the served line (the full stream is 2000)")
+    ap.add_argument("--serve-rate", type=float, default=50.0,
+                    help="Poisson arrival rate of the served line, requests/s (the workload's is 50)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -535,7 +537,8 @@ def main():
         print(json.dumps(precision_report()), flush=True)
         return
     if args.serve_only:
-        print(json.dumps(served_run(args.serve_count, device_setup=not args.serve_host_setup)),
+        print(json.dumps(served_run(args.serve_count, rate=args.serve_rate,
+                                    device_setup=not args.serve_host_setup)),
               flush=True)
         return
     dev_name = tlm.device_name(0)
@@ -611,7 +614,7 @@ def main():
     if not args.no_batch and want("batched_256"):
         extra["batched_256"] = batched_run(256)
     if not args.no_batch and want("served"):
-        extra[f"served_{args.serve_count}"] = served_run(args.serve_count,
+        extra[f"served_{args.serve_count}"] = served_run(args.serve_count, rate=args.serve_rate,
                                                          device_setup=not args.serve_host_setup)
 
     cpu = None
