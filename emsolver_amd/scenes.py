@@ -122,6 +122,16 @@ def free_space_cube(n=64, steps=1000, probe_offset=10) -> Scene:
     return s
 
 
+def add_plane_wave(scene: Scene, prop_axis: int, plane_index: int, pol_axis: int, amp: float):
+    """Soft sources on the AIR cells of the plane co[prop_axis] = plane_index,
+    in cell order, as TlmSolver::addPlaneWave (TlmSolver.cpp:394-407)."""
+    idx = np.arange(scene.ncell, dtype=np.int64)
+    co = (idx % scene.nx, (idx // scene.nx) % scene.ny, idx // (scene.nx * scene.ny))
+    cells = idx[(co[prop_axis] == plane_index) & (scene.mat == MAT_AIR)]
+    scene.sources.extend((int(c), int(pol_axis), float(amp)) for c in cells)
+    return len(cells)
+
+
 def box_list(nx, ny, nz, nboxes, rng, side=(16, 96), first_id=MAT_DIEL0):
     """`nboxes` seeded axis-aligned cell boxes [(lo, hi, id)] (later boxes
     overwrite), ids first_id .. first_id+nboxes-1, with (epsr, sigma)."""

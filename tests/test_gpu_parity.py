@@ -84,6 +84,17 @@ def test_thin_grids_bitexact(shape, rho):
     _check(sc)
 
 
+@pytest.mark.parametrize("prop_axis", [0, 2])
+def test_plane_wave_sources_bitexact(prop_axis):
+    """A plane-wave source (TlmSolver::addPlaneWave): one soft source on every
+    AIR cell of a plane, over dielectric boxes and the ground plane."""
+    sc = scenes.dielectric_boxes(n=40, nboxes=4, steps=50, nprobes=6, nfreq=3, seed=5)
+    sc.sources = []
+    nsrc = scenes.add_plane_wave(sc, prop_axis, 12, 1 if prop_axis == 0 else 0, 0.8)
+    assert nsrc > 500
+    _check(sc)
+
+
 def test_no_dielectric_bitexact():
     sc = _random_scene(20, 12, 16, seed=5, diel=False, steps=30)
     _check(sc)

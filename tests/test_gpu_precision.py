@@ -98,6 +98,16 @@ def test_thin_grids_bitexact(shape):
     _check16(dataclasses.replace(sc, pulse="bf16", fp32_halo=-1))
 
 
+def test_plane_wave_sources_bitexact():
+    """Many soft sources in one z-slab: that slab and its halo in FP32, the
+    rest 16-bit."""
+    sc = scenes.dielectric_boxes(n=40, nboxes=4, steps=50, nprobes=6, nfreq=3, seed=5)
+    sc.sources = []
+    assert scenes.add_plane_wave(sc, 2, 12, 0, 0.8) > 500
+    _check16(dataclasses.replace(sc, pulse="f16", fp32_halo=2))
+    _check16(dataclasses.replace(sc, pulse="bf16", fp32_halo=-1))
+
+
 def test_dipole_port_bitexact():
     sc = scenes.dipole_scene(24, 22, 40, length_frac=0.5, steps=150, background_boxes=3, seed=2)
     sc.probes = np.array([sc.cell(3, 4, 5), sc.cell(20, 11, 30), sc.cell(12, 11, 20)], np.int64)

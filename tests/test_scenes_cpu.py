@@ -56,3 +56,18 @@ def test_summarize():
     assert s["requests"] == 5 and s["p50_latency_s"] == np.percentile(lat, 50)
     assert s["cell_updates"] == 5 * 1000 * 10
     assert abs(s["gcell_s"] - 5e4 / (9.0 - 0.0) / 1e9) < 1e-15
+
+
+def test_plane_wave_sources_are_the_air_cells_of_the_plane():
+    """add_plane_wave follows TlmSolver::addPlaneWave (TlmSolver.cpp:394-407):
+    the AIR cells with co[prop_axis] = plane_index, in cell order."""
+    sc = scenes.dielectric_boxes(n=20, nboxes=3, steps=10, nprobes=2, nfreq=1, seed=2)
+    for axis in range(3):
+        sc.sources = []
+        n = scenes.add_plane_wave(sc, axis, 7, (axis + 1) % 3, 0.5)
+        want = [sc.cell(*co) for co in
+                [(i, j, k) for k in range(sc.nz) for j in range(sc.ny) for i in range(sc.nx)]
+                if co[axis] == 7 and sc.mat[sc.cell(*co)] == scenes.MAT_AIR]
+        assert n == len(want) > 0
+        assert [c for c, _, _ in sc.sources] == want
+        assert all(p == (axis + 1) % 3 and a == 0.5 for _, p, a in sc.sources)
