@@ -95,6 +95,24 @@ def test_plane_wave_sources_bitexact(prop_axis):
     _check(sc)
 
 
+def test_multi_cell_ports_bitexact():
+    """Ports over several cells (TlmSolver::addPort): a 2 x 3 patch fed along
+    z and a row of 4 fed along y that shares a cell with a soft source, so
+    the port sums run over many cells, the loop current is taken round the
+    middle cell, and one cell takes a source's and a port's additions in the
+    reference's order (sources, then ports)."""
+    sc = _random_scene(24, 20, 18, seed=31, rho=0.37, steps=60, random_state=False)
+    m = sc.mat
+    patch = [sc.cell(i, j, 9) for j in (8, 9) for i in (10, 11, 12)]
+    row = [sc.cell(i, 14, 5) for i in range(4, 8)]
+    m[patch + row] = scenes.MAT_AIR
+    sc.sources = [(row[2], 1, 0.6), (sc.cell(3, 3, 3), 2, 1.0)]
+    m[sc.cell(3, 3, 3)] = scenes.MAT_AIR
+    sc.ports = [(patch, 2, 1.0), (row, 1, 0.5)]
+    g, o = _check(sc)
+    assert np.abs(o.ports).max() > 0
+
+
 def test_no_dielectric_bitexact():
     sc = _random_scene(20, 12, 16, seed=5, diel=False, steps=30)
     _check(sc)
