@@ -711,7 +711,7 @@ int tlm_upload(tlm_ctx *c, const uint8_t *mat, const float *yhat, const float *g
         // blocks whose 64 codes are all 0 (the slab padding is coded PEC, so a
         // block with padding never qualifies)
         const size_t nblk = (size_t)c->P / tlm::kWave;
-        std::vector<uint8_t> ua(nblk, 0);
+        std::vector<uint8_t> ua((nblk + 3) / 4 * 4, 0); // padded for block_flag's dword loads
         for (size_t b = 0; b < nblk; ++b)
         {
             const uint16_t *q = &code[b * tlm::kWave];

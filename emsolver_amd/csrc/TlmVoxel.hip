@@ -1346,7 +1346,8 @@ int tlm_upload_mat(tlm_ctx *c, const float *yhat, const float *ghat, int nmat)
                                                                        c->nxy, c->Pp, c->code);
     HIPCHK(c, hipGetLastError());
     const long long nblk = c->P / tlm::kWave;
-    if ((rc = upload(c, &c->uair, (const uint8_t *)nullptr, (size_t)nblk)))
+    // padded to a multiple of 4 bytes for the step's dword loads of the flags
+    if ((rc = upload(c, &c->uair, (const uint8_t *)nullptr, (size_t)((nblk + 3) / 4 * 4))))
         return rc;
     k_uair<<<(unsigned)((nblk + kThreads - 1) / kThreads), kThreads, 0, c->stream>>>(c->code, nblk,
                                                                                      c->uair);

@@ -32,7 +32,8 @@ struct tlm_ctx
     char *A = nullptr, *B = nullptr;
     float *Vs = nullptr;
     uint16_t *code = nullptr; // cell code: material + neighbour-PEC bits
-    uint8_t *uair = nullptr;  // per 64-cell block: all codes 0 (air, no PEC neighbour)
+    uint8_t *uair = nullptr;  // per 64-cell block: all codes 0 (air, no PEC neighbour);
+                              // allocated to a multiple of 4 bytes
     float4 *props = nullptr;
     int nmat = 0;
     bool stubs = false, uploaded = false;

@@ -74,7 +74,8 @@ struct StepArgs
     char *B;              // incident pulses of step n+1
     float *Vs;            // stubs: 3 planes of P floats
     const uint16_t *code; // cell code (material + neighbour-PEC bits)
-    const uint8_t *uair;  // per 64-cell block (slab-major): 1 if all its codes are 0
+    const uint8_t *uair;  // per 64-cell block (slab-major): 1 if all its codes are 0;
+                          // padded to a multiple of 4 bytes (block_flag)
     const float4 *props;  // per dielectric id: (yh, gh, den, 0)
     double *epart;        // energy partial per wave task
     // 16-bit formats only
@@ -122,6 +123,21 @@ enum : unsigned
     NB_ZM = 1u << 12,
     NB_ZP = 1u << 13
 };
+
+// Flag byte b of the per-block flag array, read as the aligned dword holding
+// it through the constant address space: the index is wave-uniform and the
+// array is not written while a step runs, so this is one scalar load whose
+// result feeds a scalar branch (the array is padded to a multiple of 4
+// bytes). A byte load would be a vector load, and the wave would wait on the
+// vector memory counter for it before it can issue the cell-code load. With
+// the scalar load the 512^3 steps run 1 % faster (2.762 against 2.795 ms
+// with 40 % stub cells, 2.268 against 2.288 ms in plain air).
+__device__ __forceinline__ unsigned block_flag(const uint8_t *flags, size_t b)
+{
+    typedef const unsigned __attribute__((address_space(4))) *cword;
+    const unsigned w = *(cword)(const void *)(flags + (b & ~(size_t)3));
+    return (w >> ((unsigned)(b & 3u) * 8u)) & 0xFFu;
+}
 
 // Workgroups are dispatched to the 8 XCDs round robin. Each XCD gets one
 // contiguous range of task groups, walked by its own workgroups in a
@@ -203,7 +219,7 @@ __device__ __forceinline__ void strip_task(const StepArgs &a, int task, int lane
         bool pec = true;
         unsigned code = 0;
         // blocks of plain air cells with no PEC neighbour skip the code load
-        const bool uair = a.uair[(size_t)k * a.strips + t] != 0;
+        const bool uair = block_flag(a.uair, (size_t)k * a.strips + t) != 0u;
         if (act)
         {
             float v[12];
