@@ -139,6 +139,16 @@ __device__ __forceinline__ unsigned block_flag(const uint8_t *flags, size_t b)
     return (w >> ((unsigned)(b & 3u) * 8u)) & 0xFFu;
 }
 
+// A float at a wave-uniform address that no kernel of the step writes (the
+// per-slab scales), read through the constant address space so that it is a
+// scalar load: a vector load there would make the wave wait for all of its
+// earlier stores before it could use the value (stores share the vector
+// memory counter).
+__device__ __forceinline__ float ld_uniform(const float *p)
+{
+    return *(const float __attribute__((address_space(4))) *)(const void *)p;
+}
+
 // Workgroups are dispatched to the 8 XCDs round robin. Each XCD gets one
 // contiguous range of task groups, walked by its own workgroups in a
 // grid-stride loop, so neighbouring strips share an L2 (which keeps the
@@ -422,7 +432,7 @@ __device__ __forceinline__ Slab16 slab16(const StepArgs &a, int k, int f)
     const bool h = __builtin_amdgcn_readfirstlane((int)(k < a.h0 || k >= a.h1)) != 0; // per wave
     const unsigned es = h ? 4u : 8u;
     const size_t base = slab16_base(a.Pp, k, a.h0, a.h1);
-    return {base, blk16((unsigned)f, es), es, h, a.dsc[k], a.esc[k]};
+    return {base, blk16((unsigned)f, es), es, h, ld_uniform(a.dsc + k), ld_uniform(a.esc + k)};
 }
 template <int PF, bool H16>
 __device__ __forceinline__ void ldpair(const char *p, unsigned off, const Slab16 &s, float &x,
