@@ -1349,6 +1349,7 @@ int tlm_upload_mat(tlm_ctx *c, const float *yhat, const float *ghat, int nmat)
     // padded to a multiple of 4 bytes for the step's dword loads of the flags
     if ((rc = upload(c, &c->uair, (const uint8_t *)nullptr, (size_t)((nblk + 3) / 4 * 4))))
         return rc;
+    c->fold_marked = false;
     k_uair<<<(unsigned)((nblk + kThreads - 1) / kThreads), kThreads, 0, c->stream>>>(c->code, nblk,
                                                                                      c->uair);
     HIPCHK(c, hipGetLastError());

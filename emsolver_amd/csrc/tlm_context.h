@@ -22,7 +22,7 @@ struct tlm_ctx
     long long nxy = 0, Pp = 0, P = 0; // slab pitch roundup(nx*ny, 64), plane stride Pp*nz
     float dl = 0.f, rho = 0.f;
     // launch shape of the fused kernel
-    tlm_tuning tune = {0, 1, 4, 2, 0}; // non-temporal stores chosen per grid (automatic)
+    tlm_tuning tune = {0, 1, 4, 2, 0, 2}; // non-temporal stores and folding chosen per grid
     int strips = 0, zc = 0, nzc = 0, tasks = 0, groups = 0, blocks = 0, wpb = 4;
     size_t epart_cap = 0;
     // state
@@ -43,8 +43,10 @@ struct tlm_ctx
     int *ubeg = nullptr, *spol = nullptr;
     float *samp = nullptr;
     int nuniq = 0;
+    bool fold_marked = false; // FP32: blocks of the source cells carry flag bit 1
     // schedule
     float *sv = nullptr, *cph = nullptr, *sph = nullptr;
+    std::vector<float> sv_host; // the waveform values, for the folded sources
     int nsteps = 0, settle = 0;
     // monitors
     long long *fcell = nullptr;

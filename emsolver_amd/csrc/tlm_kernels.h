@@ -68,13 +68,60 @@ template <int PF> __device__ __forceinline__ unsigned short enc16(float v)
     return (unsigned short)(u >> 16);
 }
 
+// Soft sources folded into the step (FP32 contexts with at most
+// kFoldMax source cells): the step and its monitor workgroups add step n's
+// sources to the incident pulses they load, in registers, instead of a
+// separate injection launch writing them into A first. ninj = 0: none.
+constexpr int kFoldMax = 32;
+struct Fold
+{
+    const long long *cell = nullptr; // distinct source cells (padded index), ascending
+    const int *beg = nullptr;        // sources of cell u: [beg[u], beg[u+1]), list order
+    const int *pol = nullptr;
+    const float *amp = nullptr;
+    int ninj = 0;
+    float sv = 0.0f;                 // waveform value of the step
+};
+
+// Adds the folded sources of cell c to its 12 incident pulses v, in list
+// order: the same float additions k_inject would store (TlmSolver.cpp:
+// 487-505), so every later use sees the same values.
+__device__ __forceinline__ void fold_sources(const Fold &f, long long c, float v[12])
+{
+    for (int u = 0; u < f.ninj; ++u)
+    {
+        if (f.cell[u] != c)
+            continue;
+        for (int s = f.beg[u]; s < f.beg[u + 1]; ++s)
+        {
+            const int pol = f.pol[s];
+            const float qv = 0.5f * f.amp[s] * f.sv;
+#pragma unroll
+            for (int pp = 0; pp < 3; ++pp)
+            {
+                if (pp != pol)
+                    continue;
+#pragma unroll
+                for (int axis = 0; axis < 3; ++axis)
+                {
+                    if (axis == pp)
+                        continue;
+                    v[scn::port_of(axis, 0, pp)] = v[scn::port_of(axis, 0, pp)] + qv;
+                    v[scn::port_of(axis, 1, pp)] = v[scn::port_of(axis, 1, pp)] + qv;
+                }
+            }
+        }
+    }
+}
+
 struct StepArgs
 {
     const char *A;        // incident pulses of step n: 12 planes of PB bytes
     char *B;              // incident pulses of step n+1
     float *Vs;            // stubs: 3 planes of P floats
     const uint16_t *code; // cell code (material + neighbour-PEC bits)
-    const uint8_t *uair;  // per 64-cell block (slab-major): 1 if all its codes are 0;
+    const uint8_t *uair;  // per 64-cell block (slab-major): bit 0 all its codes are 0,
+                          // bit 1 it holds a folded source cell;
                           // padded to a multiple of 4 bytes (block_flag)
     const float4 *props;  // per dielectric id: (yh, gh, den, 0)
     double *epart;        // energy partial per wave task
@@ -94,6 +141,7 @@ struct StepArgs
     int tasks;           // strips * nzc
     int groups;          // task groups (one per workgroup's worth of waves)
     float rho;
+    Fold fold; // FP32: soft sources added where the pulses are loaded
 };
 
 // Loads and stores through a uniform plane base (SGPR pair) and a 32-bit
@@ -197,7 +245,7 @@ __device__ __forceinline__ unsigned blk_off(unsigned g)
 // cell PEC (value kept, times rho on an outer face), outer face (rho * out),
 // neighbour PEC (-out). This is connect(), TlmSolver.cpp:563-583.
 // One wave task: a 64-cell strip of slabs [k0, k1).
-template <bool STUBS, bool ENERGY, bool NT, int PF>
+template <bool STUBS, bool ENERGY, bool NT, int PF, bool FOLD = false>
 __device__ __forceinline__ void strip_task(const StepArgs &a, int task, int lane, double &esum)
 {
     static_assert(PF == PF_F32, "the 16-bit formats use strip_task16");
@@ -229,7 +277,9 @@ __device__ __forceinline__ void strip_task(const StepArgs &a, int task, int lane
         bool pec = true;
         unsigned code = 0;
         // blocks of plain air cells with no PEC neighbour skip the code load
-        const bool uair = block_flag(a.uair, (size_t)k * a.strips + t) != 0u;
+        // bit 0: plain-air block (no code load); bit 1: block holds a folded source
+        const unsigned fl = block_flag(a.uair, (size_t)k * a.strips + t);
+        const bool uair = (fl & 1u) != 0u;
         if (act)
         {
             float v[12];
@@ -242,6 +292,8 @@ __device__ __forceinline__ void strip_task(const StepArgs &a, int task, int lane
                 v[p] = ldo(Ak + p * 256, lo);
             if (STUBS && !uair)
                 code = ldc(a.code, mo);
+            if (FOLD && (fl & 2u))
+                fold_sources(a.fold, (long long)mo, v);
             const unsigned m = code & 0xFFu;
             pec = (m == 1u);
             if (ENERGY)
@@ -682,6 +734,7 @@ struct PulseView
     int h0, h1;        // 16-bit formats: the FP32 slabs [h0, h1)
     const float *sc;  // per slab 2^E (current step)
     const float *isc; // per slab 2^-E
+    Fold fold;        // monitors of an FP32 step with folded sources
 };
 // FP32 contexts: 64-cell blocks of 12 ports per slab. 16-bit contexts: 64-cell
 // blocks of 6 pairs (pair p/2, half p&1), two codes per cell and pair in a
@@ -891,6 +944,18 @@ __device__ __forceinline__ void gather12(const PulseView &pv, long long c, float
 #pragma unroll
     for (int p = 0; p < 12; ++p)
         v[p] = pv_get(pv, p, c);
+    if (pv.fold.ninj > 0)
+        fold_sources(pv.fold, c, v);
+}
+
+// Marks the blocks holding folded source cells (flag bit 1); one thread, so
+// two cells of one block cannot race on the flag byte.
+static __global__ void k_mark_fold(const long long *cell, int n, uint8_t *flags)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0)
+        return;
+    for (int u = 0; u < n; ++u)
+        flags[cell[u] / kWave] |= 2u;
 }
 
 // Port record of step n (TlmSolver.cpp:677-703): sequential float sum of
@@ -1063,7 +1128,7 @@ __device__ __forceinline__ void monitor_item(const MonitorArgs &a, int id)
 
 // The fused step: scatter + connect of every cell (strip_task / strip_task16)
 // and, in workgroups [mon_from, gridDim.x), the monitor records.
-template <bool STUBS, bool ENERGY, bool NT, int PF>
+template <bool STUBS, bool ENERGY, bool NT, int PF, bool FOLD = false>
 __global__ void __launch_bounds__(256) k_step(StepArgs a, MonitorArgs m, int mon_from)
 {
     // workgroups from mon_from on take the monitor records of this step
@@ -1085,7 +1150,7 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a, MonitorArgs m, int mon
         if (task < a.tasks)
         {
             if constexpr (PF == PF_F32)
-                strip_task<STUBS, ENERGY, NT, PF>(a, task, lane, esum);
+                strip_task<STUBS, ENERGY, NT, PF, FOLD>(a, task, lane, esum);
             else
                 strip_task16<STUBS, ENERGY, PF>(a, task, lane, esum);
         }

@@ -221,6 +221,11 @@ typedef struct tlm_tuning {
     int32_t nontemporal; /* non-temporal stores of the next-step pulses: 0 no, 1 yes, 2 when the
                             grid has dielectric cells (default) */
     int32_t grid_blocks; /* > 0: persistent grid of that many workgroups; 0 = one per task group */
+    int32_t fold;        /* FP32 with at most 32 source cells: add the soft sources inside the
+                            step instead of a separate injection launch. 0 no, 1 yes, 2 (default)
+                            for grids without stub cells and for grids with them of at most
+                            4 M cells, where the launch it saves weighs more than the stub
+                            kernel's own cost of the folded code */
 } tlm_tuning;
 
 int tlm_set_tuning(tlm_ctx *ctx, const tlm_tuning *t);

@@ -40,6 +40,17 @@ def test_256_scene_bitexact_first_steps():
     _check(_near_probes(sc, [(5, 0, 0), (0, 7, 3), (-9, -4, 0), (2, 2, -11)]))
 
 
+def test_256_scene_bitexact_first_steps_folded_sources():
+    """The same with the source added inside the step (tuning fold = 1),
+    which by default only grids of at most 4 M cells do."""
+    from tests.test_gpu_parity import _run_tuned
+    sc = _near_probes(scenes.dielectric_boxes(n=256, nboxes=8, steps=40, nprobes=16, nfreq=32,
+                                              seed=1), [(5, 0, 0), (0, 7, 3), (0, 0, 0)])
+    print(f"[full] {sc.name} with folded sources", flush=True)
+    g = _run_tuned(sc, {"fold": 1})
+    helpers.compare_results(g, helpers.oracle_run(sc), sc)
+
+
 def test_512_scene_bitexact_first_steps_fp32():
     sc = scenes.dielectric_boxes(n=512, nboxes=64, steps=6, nprobes=16, nfreq=4, seed=1)
     _check(_near_probes(sc, [(2, 0, 0), (0, 3, 0), (-1, -2, 1)]))

@@ -160,8 +160,8 @@ def test_free_space_64_full_run_tolerances():
 
 
 TUNINGS = [
-    {"kernel": 0, "planes": 1, "block_waves": 4, "nontemporal": 0, "grid_blocks": 0},
-    {"kernel": 0, "planes": 0, "block_waves": 0, "nontemporal": 1, "grid_blocks": 0},
+    {"kernel": 0, "planes": 1, "block_waves": 4, "nontemporal": 0, "grid_blocks": 0, "fold": 0},
+    {"kernel": 0, "planes": 0, "block_waves": 0, "nontemporal": 1, "grid_blocks": 0, "fold": 1},
     {"kernel": 0, "planes": 3, "block_waves": 1, "nontemporal": 0, "grid_blocks": 5},
     {"kernel": 0, "planes": 1, "block_waves": 2, "nontemporal": 1, "grid_blocks": 17},
     {"kernel": 0, "planes": 64, "block_waves": 4, "nontemporal": 0, "grid_blocks": 8},
@@ -178,7 +178,7 @@ def _run_tuned(sc, tuning, steps=None):
         return tlm.collect(dev, sc, steps)
 
 
-@pytest.mark.parametrize("tuning", TUNINGS, ids=lambda t: "z{planes}_w{block_waves}_nt{nontemporal}_g{grid_blocks}".format(**t))
+@pytest.mark.parametrize("tuning", TUNINGS, ids=lambda t: "z{planes}_w{block_waves}_nt{nontemporal}_g{grid_blocks}_f{}".format(t.get("fold", 2), **t))
 @pytest.mark.parametrize("shape", [(12, 9, 11), (300, 5, 7), (63, 7, 5), (2000, 2, 3)])
 def test_tunings_bitexact(shape, tuning):
     """Every launch shape gives the oracle's bits, including rows that start
@@ -199,6 +199,21 @@ def test_plane_and_frames_bitexact():
     _check(sc)
     sc.plane_axis, sc.plane_index = 0, 99   # clamped to the last x plane
     _check(sc.with_steps(70))
+
+
+@pytest.mark.parametrize("fold", [0, 1])
+def test_folded_and_launched_injection_agree(fold):
+    """Soft sources and a port added inside the step (fold 1) or by the
+    injection launch (fold 0): the oracle's bits either way, with probes
+    and a port on the source cells themselves, PEC neighbours and stubs."""
+    sc = _random_scene(30, 11, 9, seed=44, rho=-1.0, steps=30)
+    port = [sc.cell(15, 5, 4), sc.cell(15, 5, 5)]   # interior: the current loop stays in the grid
+    sc.mat[port] = scenes.MAT_AIR
+    sc.ports = [(port, 2, 0.9)]
+    sc.probes = np.array([c for c, _, _ in sc.sources] + [port[0]], np.int64)
+    print(f"[parity] fold {fold}", flush=True)
+    g = _run_tuned(sc, {"fold": fold})
+    helpers.compare_results(g, helpers.oracle_run(sc), sc)
 
 
 @pytest.mark.parametrize("every", [1, 3])
