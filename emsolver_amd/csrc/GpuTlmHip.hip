@@ -3,8 +3,9 @@
 // Replaces the OpenGL set-up and per-step dispatch of GpuTlm::run
 // (GpuTlm.cpp:532-884 in the reference) with HIP: one stream per context,
 // device-resident per-step tables, two launches per step (inject, then the
-// fused scatter/connect whose extra workgroups take the monitor records) and
-// no host synchronisation inside a run.
+// fused scatter/connect whose extra workgroups take the monitor records; one
+// where the step adds the sources itself) and no host synchronisation inside
+// a run.
 #include "tlm_context.h"
 
 #include <hip/hip_ext.h>

@@ -22,6 +22,8 @@
 //
 // One step is (TlmSolver::step, TlmSolver.cpp:485-513):
 //   k_inject  : soft sources add 0.5*amp*sv(n) into A        (:487-505)
+//               (FP32 grids with few source cells: k_step adds them itself
+//               where it loads the pulses, fold_sources)
 //   k_step    : scatter every cell from A and deliver each reflected
 //               pulse to the slot it reaches after connect, writing B
 //               (scatter :516-548 and connect :551-613 fused); optionally
