@@ -216,6 +216,19 @@ def test_folded_and_launched_injection_agree(fold):
     helpers.compare_results(g, helpers.oracle_run(sc), sc)
 
 
+def test_fold_tuning_rules():
+    """tuning.fold: 0..2 accepted and read back (default 2), anything else
+    refused."""
+    sc = _random_scene(20, 9, 7, seed=5, diel=False, steps=4)
+    with tlm.open_scene(sc) as dev:
+        assert dev.get_tuning()["fold"] == 2
+        for f in (0, 1, 2):
+            dev.set_tuning(fold=f)
+            assert dev.get_tuning()["fold"] == f
+        with pytest.raises(tlm.TlmError, match="tlm_set_tuning"):
+            dev.set_tuning(fold=3)
+
+
 @pytest.mark.parametrize("every", [1, 3])
 def test_timed_steps_equal_untimed(every):
     """tlm_time_steps (events on every `every`-th step launch) runs the same
