@@ -4,7 +4,8 @@
 
 compiles ``emsolver_amd/lib/libemsolver_tlm.so`` from the HIP sources under
 ``csrc/`` and the C++ drop-in glue under ``host/`` (TlmSolver facade and
-RunGpuTlm), plus the self-test program ``tests/selftest/tlm_selftest``.
+RunGpuTlm), plus the test programs ``tests/selftest/tlm_selftest`` and
+``tests/selftest/tlm_stop``.
 
 Flags that matter:
   -ffp-contract=off  no fused multiply-add: every a*b+c rounds twice, as the
@@ -55,14 +56,15 @@ def build(verbose: bool = False) -> str:
     tmp = lib + ".tmp"
     _run([HIPCC, *DEVICE, *COMMON, "-shared", "-o", tmp, *srcs], verbose)
     os.replace(tmp, lib)
-    # the self-test program (test infrastructure, lives with the tests)
-    st = os.path.join(ROOT, "tests", "selftest", "tlm_selftest.cpp")
-    if os.path.exists(st):
-        exe = os.path.join(ROOT, "tests", "selftest", "tlm_selftest")
-        _run([HIPCC, *COMMON, "-o", exe + ".tmp", st, f"-L{LIB_DIR}", "-lemsolver_tlm",
-              f"-Wl,-rpath,{LIB_DIR}", "-Wl,-rpath,$ORIGIN/../../emsolver_amd/lib", "-pthread",
-              "-ldl"], verbose)
-        os.replace(exe + ".tmp", exe)
+    # the self-test and cancellation programs (test infrastructure, live with the tests)
+    for name in ("tlm_selftest", "tlm_stop"):
+        st = os.path.join(ROOT, "tests", "selftest", name + ".cpp")
+        if os.path.exists(st):
+            exe = os.path.join(ROOT, "tests", "selftest", name)
+            _run([HIPCC, *COMMON, "-o", exe + ".tmp", st, f"-L{LIB_DIR}", "-lemsolver_tlm",
+                  f"-Wl,-rpath,{LIB_DIR}", "-Wl,-rpath,$ORIGIN/../../emsolver_amd/lib", "-pthread",
+                  "-ldl"], verbose)
+            os.replace(exe + ".tmp", exe)
     return lib
 
 

@@ -624,6 +624,11 @@ int tlm_create(const tlm_grid_desc *d, tlm_ctx **out)
         if (pp * (long long)d->nz * 4 > (1LL << 32) - (1LL << 20))
             return fail(nullptr, TLM_ERR_UNSUPPORTED,
                         "grid above ~1.07e9 cells: a pulse plane exceeds 32-bit byte offsets");
+        // the FP32 step addresses a slab's 12 ports (48 bytes per cell) with
+        // 32-bit byte offsets from the slab base
+        if (pp * 48 > (1LL << 32) - (1LL << 20))
+            return fail(nullptr, TLM_ERR_UNSUPPORTED,
+                        "z-slab above ~89e6 cells: its pulses exceed 32-bit byte offsets");
     }
     if (!(d->dl > 0.0f))
         return fail(nullptr, TLM_ERR_ARG, "cell size must be positive");

@@ -16,13 +16,40 @@
 #include "TlmSolver.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <utility>
 #include <vector>
 
 class GpuTlm
 {
 public:
     static bool run(TlmSolver &s, std::string &msg);
+
+private:
+    // The running FP64 probe DFT is an addition of this build's facade
+    // (TlmSolver::dftFreqs, FieldProbe::dftRe / dftIm). Against the
+    // reference's own TlmSolver.h, which has neither, these overloads select
+    // the versions without it, so this file compiles against both headers.
+    // (Members of a friend class may name TlmSolver's private members.)
+    template <class S>
+    static auto dft_freqs(const S &s, int) -> decltype(s.dftFreqs, std::vector<double>())
+    {
+        return std::vector<double>(s.dftFreqs.begin(), s.dftFreqs.end());
+    }
+    template <class S> static std::vector<double> dft_freqs(const S &, long)
+    {
+        return {};
+    }
+    typedef std::pair<std::vector<double> *, std::vector<double> *> DftPtrs;
+    template <class P> static auto probe_dft(P &p, int) -> decltype(p.dftRe, p.dftIm, DftPtrs())
+    {
+        return {&p.dftRe, &p.dftIm};
+    }
+    template <class P> static DftPtrs probe_dft(P &, long)
+    {
+        return {nullptr, nullptr};
+    }
 };
 
 bool RunGpuTlm(TlmSolver &s, std::string &msg)
@@ -48,7 +75,14 @@ bool failed(std::string &msg, const char *what, tlm_ctx *c)
     return false;
 }
 
-const int kBlockSteps = 64; // steps per queued block between host checks
+// Steps are queued in blocks; between blocks the host publishes energy and
+// |Js| for the UI and checks stopFlag. A block is sized to take about
+// kBlockMs of device time (measured on the blocks already run), at most
+// kBlockSteps steps, so a stop request is honoured within about kBlockMs on
+// any grid (the reference's loops test stopFlag every step:
+// TlmSolver.cpp:459, GpuTlm.cpp:714).
+const int kBlockSteps = 64;
+const double kBlockMs = 20.0;
 
 } // namespace
 
@@ -159,7 +193,9 @@ bool GpuTlm::run(TlmSolver &s, std::string &msg)
             return failed(msg, "tlm_set_ports", c);
     }
     const int npr = (int)s.probes.size();
-    const int nfreq = (int)s.dftFreqs.size();
+    const std::vector<double> freqs = dft_freqs(s, 0);
+    const int nfreq = npr > 0 ? (int)freqs.size() : 0;
+    std::vector<double> ct, st; // FP64 DFT phasors of steps [0, total)
     if (npr > 0)
     {
         std::vector<int64_t> pc(npr);
@@ -169,14 +205,13 @@ bool GpuTlm::run(TlmSolver &s, std::string &msg)
             return failed(msg, "tlm_set_probes", c);
         if (nfreq > 0)
         {
-            if (start != 0)
-            {
-                msg = "the probe DFT needs the run to start at step 0";
-                return false;
-            }
-            std::vector<double> ct((size_t)total * nfreq), st((size_t)total * nfreq);
-            tlm_dft_tables(s.dftFreqs.data(), nfreq, (double)s.dt, total, ct.data(), st.data());
-            if (tlm_set_probe_dft(c, ct.data(), st.data(), nfreq) != TLM_OK)
+            ct.resize((size_t)total * nfreq);
+            st.resize((size_t)total * nfreq);
+            tlm_dft_tables(freqs.data(), nfreq, (double)s.dt, total, ct.data(), st.data());
+            // the device sums a run from step 0; a run resumed at a later step
+            // continues the solver's sums on the host (same FP64 terms in the
+            // same step order)
+            if (start == 0 && tlm_set_probe_dft(c, ct.data(), st.data(), nfreq) != TLM_OK)
                 return failed(msg, "tlm_set_probe_dft", c);
         }
     }
@@ -191,11 +226,16 @@ bool GpuTlm::run(TlmSolver &s, std::string &msg)
     int done = start;
     std::vector<int32_t> esteps;
     std::vector<double> evals;
+    int block = 1; // the first block times one step
     while (done < total && !s.stopFlag)
     {
-        const int nb = std::min(kBlockSteps, total - done);
+        const int nb = std::min(block, total - done);
+        const auto t0 = std::chrono::steady_clock::now();
         if (tlm_run(c, done, nb) != TLM_OK || tlm_sync(c) != TLM_OK)
             return failed(msg, "tlm_run", c);
+        const double ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        block = std::max(1, std::min(kBlockSteps, (int)(kBlockMs * nb / std::max(ms, 1e-3))));
         done += nb;
         s.curStep = done;
         const int ne = tlm_read_energy(c, nullptr, nullptr, 0);
@@ -220,24 +260,35 @@ bool GpuTlm::run(TlmSolver &s, std::string &msg)
             std::vector<float> d4((size_t)nf * 4);
             if (tlm_read_js(c, s.jsInstant.data(), d4.data()) != TLM_OK)
                 return failed(msg, "tlm_read_js", c);
+            // the device sums start from 0 at `start`: added to the solver's
+            // sums (zero after setup), so a resumed run continues them and
+            // dftSamples counts the same steps (TlmSolver.cpp:467-482 divides
+            // one by the other)
             for (int f = 0; f < nf; ++f)
             {
-                s.jsDftRe[f * 2] = d4[f * 4];
-                s.jsDftIm[f * 2] = d4[f * 4 + 1];
-                s.jsDftRe[f * 2 + 1] = d4[f * 4 + 2];
-                s.jsDftIm[f * 2 + 1] = d4[f * 4 + 3];
+                s.jsDftRe[f * 2] += d4[f * 4];
+                s.jsDftIm[f * 2] += d4[f * 4 + 1];
+                s.jsDftRe[f * 2 + 1] += d4[f * 4 + 2];
+                s.jsDftIm[f * 2 + 1] += d4[f * 4 + 3];
             }
         }
         const int dftSteps = std::max(0, done - std::max(start, s.config.settleSteps));
         s.dftSamples += dftSteps;
         if (nh > 0)
         {
-            if (tlm_read_huygens(c, s.huyDft.data()) != TLM_OK)
+            std::vector<float> h8(s.huyDft.size());
+            if (tlm_read_huygens(c, h8.data()) != TLM_OK)
                 return failed(msg, "tlm_read_huygens", c);
+            for (size_t q = 0; q < h8.size(); ++q)
+                s.huyDft[q] += h8[q];
             s.huySamples += dftSteps;
         }
-        s.energySteps.clear();
-        s.energyVals.clear();
+        // records of earlier runs (steps before `start`) are kept
+        while (!s.energySteps.empty() && s.energySteps.back() >= start)
+        {
+            s.energySteps.pop_back();
+            s.energyVals.pop_back();
+        }
         for (size_t q = 0; q < esteps.size(); ++q)
         {
             s.energySteps.push_back(esteps[q]);
@@ -291,13 +342,17 @@ bool GpuTlm::run(TlmSolver &s, std::string &msg)
         std::vector<float> rec((size_t)(done - start) * np * 2);
         if (tlm_read_ports(c, start, done - start, rec.data()) != TLM_OK)
             return failed(msg, "tlm_read_ports", c);
+        // records of steps [0, done) (GpuTlm.cpp:864-879 resizes to the
+        // steps run and assigns)
         for (int p = 0; p < np; ++p)
         {
             auto &port = s.portList[p];
+            port.vRec.resize(done);
+            port.iRec.resize(done);
             for (int t = 0; t < done - start; ++t)
             {
-                port.vRec.push_back(rec[((size_t)t * np + p) * 2]);
-                port.iRec.push_back(rec[((size_t)t * np + p) * 2 + 1]);
+                port.vRec[start + t] = rec[((size_t)t * np + p) * 2];
+                port.iRec[start + t] = rec[((size_t)t * np + p) * 2 + 1];
             }
         }
     }
@@ -309,24 +364,60 @@ bool GpuTlm::run(TlmSolver &s, std::string &msg)
         for (int p = 0; p < npr; ++p)
         {
             auto &pr = s.probes[p];
+            pr.ex.resize(done);
+            pr.ey.resize(done);
+            pr.ez.resize(done);
             for (int t = 0; t < done - start; ++t)
             {
-                pr.ex.push_back(rec[((size_t)t * npr + p) * 3]);
-                pr.ey.push_back(rec[((size_t)t * npr + p) * 3 + 1]);
-                pr.ez.push_back(rec[((size_t)t * npr + p) * 3 + 2]);
+                pr.ex[start + t] = rec[((size_t)t * npr + p) * 3];
+                pr.ey[start + t] = rec[((size_t)t * npr + p) * 3 + 1];
+                pr.ez[start + t] = rec[((size_t)t * npr + p) * 3 + 2];
             }
         }
         if (nfreq > 0)
         {
-            std::vector<double> re((size_t)npr * 3 * nfreq), im(re.size());
-            if (tlm_read_probe_dft(c, re.data(), im.data()) != TLM_OK)
-                return failed(msg, "tlm_read_probe_dft", c);
+            const size_t per = (size_t)3 * nfreq;
+            std::vector<double> re((size_t)npr * per), im(re.size());
+            if (start == 0)
+            {
+                if (tlm_read_probe_dft(c, re.data(), im.data()) != TLM_OK)
+                    return failed(msg, "tlm_read_probe_dft", c);
+            }
+            else
+            {
+                // continue the solver's sums over [start, done): re += x cos,
+                // im -= x sin per step, as the device kernel adds them
+                for (int p = 0; p < npr; ++p)
+                {
+                    const auto d = probe_dft(s.probes[p], 0);
+                    if (d.first && d.first->size() == per && d.second->size() == per)
+                    {
+                        std::copy(d.first->begin(), d.first->end(), re.begin() + p * per);
+                        std::copy(d.second->begin(), d.second->end(), im.begin() + p * per);
+                    }
+                }
+                for (int p = 0; p < npr; ++p)
+                    for (int comp = 0; comp < 3; ++comp)
+                        for (int q = 0; q < nfreq; ++q)
+                        {
+                            double ar = re[p * per + comp * nfreq + q], ai = im[p * per + comp * nfreq + q];
+                            for (int t = start; t < done; ++t)
+                            {
+                                const double x = (double)rec[((size_t)(t - start) * npr + p) * 3 + comp];
+                                ar += x * ct[(size_t)t * nfreq + q];
+                                ai -= x * st[(size_t)t * nfreq + q];
+                            }
+                            re[p * per + comp * nfreq + q] = ar;
+                            im[p * per + comp * nfreq + q] = ai;
+                        }
+            }
             for (int p = 0; p < npr; ++p)
             {
-                s.probes[p].dftRe.assign(re.begin() + (size_t)p * 3 * nfreq,
-                                         re.begin() + (size_t)(p + 1) * 3 * nfreq);
-                s.probes[p].dftIm.assign(im.begin() + (size_t)p * 3 * nfreq,
-                                         im.begin() + (size_t)(p + 1) * 3 * nfreq);
+                const auto d = probe_dft(s.probes[p], 0);
+                if (!d.first)
+                    continue;
+                d.first->assign(re.begin() + p * per, re.begin() + (p + 1) * per);
+                d.second->assign(im.begin() + p * per, im.begin() + (p + 1) * per);
             }
         }
     }

@@ -14,7 +14,6 @@
 namespace {
 const float kC0 = 299792458.0f;   // TlmSolver.cpp:15
 const float kEta0 = 376.730313f;  // TlmSolver.cpp:16
-const int kPol[3][2] = {{1, 2}, {0, 2}, {0, 1}};
 } // namespace
 
 void BuildPecSurfaceFaces(const VoxelGridSpec &g, const std::vector<uint8_t> &mat, uint8_t matPec,
@@ -274,7 +273,8 @@ void TlmSolver::getEnergyHistory(std::vector<int> &steps, std::vector<float> &va
 
 bool TlmSolver::computeFarField(FarFieldData &out, int nTheta, int nPhi)
 {
-    return ComputeFarFieldFromHuygens(huyFaces, huyDft, huySamples, config.f0, out, nTheta, nPhi);
+    return ComputeFarFieldFromHuygens(huyFaces, huyDft, huySamples, config.f0, nullptr, out, nTheta,
+                                      nPhi);
 }
 
 float TlmSolver::ScatterUnitarityError()
@@ -289,26 +289,22 @@ float TlmSolver::DielScatterUnitarityError(float epsr)
 
 //---------------------------------------------------------------------------
 // Near-to-far-field transform at f0 (the reference's Huygens NTFF,
-// TlmSolver.cpp:805-916, restated from the equivalence principle).
-//
-// On a face with outward normal n = s*e_a and tangential axes (t1, t2):
-//   J = n x H,   M = -n x E,
-// and with e_a x e_t1 = sa*e_t2 (sa = -1 only for a = y, whose tangential
-// pair (x, z) is anti-cyclic) the components are
-//   J_t2 = s*sa*H1, J_t1 = -s*sa*H2, M_t2 = -s*sa*E1, M_t1 = s*sa*E2
-// (H scaled by eta0 as stored). The radiation vectors sum the currents with
-// the phase exp(+j k0 r.r'); the intensity is
-//   U ~ |L_phi + N_theta|^2 + |L_theta - N_phi|^2,
-// and the directivity 4*pi*Umax / integral(U sin(theta)) by the trapezoid
-// rule on the (theta, phi) grid.
-//
-// With a HIP device present the sums run on it (tlm_far_field); the loops
-// below are the host path of a machine without one, as the reference's CPU
-// loop is the path when RunGpuTlm returns false.
+// TlmSolver.cpp:805-916), on the device: tlm_far_field sums every face's
+// equivalent currents J = n x H, M = -n x E times exp(+j k0 r.r') for each
+// direction and returns U ~ |L_phi + N_theta|^2 + |L_theta - N_phi|^2, the
+// directivity 4*pi*Umax / integral(U sin(theta)) (trapezoid rule on the
+// (theta, phi) grid) and the peak direction.
 //---------------------------------------------------------------------------
-static bool FarFieldOnDevice(const std::vector<HuyFace> &huyFaces, const std::vector<float> &huyDft,
-                             int huySamples, float f0, FarFieldData &out, int nTheta, int nPhi)
+bool ComputeFarFieldFromHuygens(const std::vector<HuyFace> &huyFaces,
+                                const std::vector<float> &huyDft, int huySamples, float f0,
+                                ThreadPool * /*pool*/, FarFieldData &out, int nTheta, int nPhi)
 {
+    if (huyFaces.empty() || huySamples == 0 || nTheta < 2 || nPhi < 2)
+        return false;
+    if (huyDft.size() < huyFaces.size() * 8)
+        return false;
+    if (tlm_device_count() <= 0)
+        return false;
     const size_t nh = huyFaces.size();
     std::vector<float> pos(3 * nh);
     std::vector<int32_t> ax(nh), sg(nh);
@@ -332,110 +328,5 @@ static bool FarFieldOnDevice(const std::vector<HuyFace> &huyFaces, const std::ve
     out.directivity = sm.directivity;
     out.peakThetaDeg = sm.peak_theta_deg;
     out.peakPhiDeg = sm.peak_phi_deg;
-    return true;
-}
-
-bool ComputeFarFieldFromHuygens(const std::vector<HuyFace> &huyFaces,
-                                const std::vector<float> &huyDft, int huySamples, float f0,
-                                FarFieldData &out, int nTheta, int nPhi)
-{
-    if (huyFaces.empty() || huySamples == 0 || nTheta < 2 || nPhi < 2)
-        return false;
-    if (huyDft.size() < huyFaces.size() * 8)
-        return false;
-    if (tlm_device_count() > 0)
-        return FarFieldOnDevice(huyFaces, huyDft, huySamples, f0, out, nTheta, nPhi);
-    const float PI = (float)M_PI;
-    const float k0 = 2.0f * PI * f0 / kC0;
-    const size_t nh = huyFaces.size();
-    struct Cur
-    {
-        Vec3 pos;
-        float jr[3], ji[3], mr[3], mi[3];
-    };
-    std::vector<Cur> cur(nh);
-    for (size_t f = 0; f < nh; ++f)
-    {
-        const HuyFace &h = huyFaces[f];
-        const float *d = &huyDft[f * 8]; // E1, E2, eta0*H1, eta0*H2 (re, im)
-        const int t1 = kPol[h.axis][0], t2 = kPol[h.axis][1];
-        const float ss = (float)h.outSign * (h.axis == 1 ? -1.0f : 1.0f);
-        Cur &c = cur[f];
-        c.pos = h.pos;
-        for (int q = 0; q < 3; ++q)
-            c.jr[q] = c.ji[q] = c.mr[q] = c.mi[q] = 0.0f;
-        c.jr[t2] = ss * d[4];
-        c.ji[t2] = ss * d[5];
-        c.jr[t1] = -ss * d[6];
-        c.ji[t1] = -ss * d[7];
-        c.mr[t2] = -ss * d[0];
-        c.mi[t2] = -ss * d[1];
-        c.mr[t1] = ss * d[2];
-        c.mi[t1] = ss * d[3];
-    }
-    out.nTheta = nTheta;
-    out.nPhi = nPhi;
-    out.U.assign((size_t)nTheta * nPhi, 0.0f);
-    for (int it = 0; it < nTheta; ++it)
-    {
-        const float th = PI * it / (nTheta - 1);
-        const float st = std::sin(th), ct = std::cos(th);
-        for (int ip = 0; ip < nPhi; ++ip)
-        {
-            const float ph = 2.0f * PI * ip / (nPhi - 1);
-            const float cp = std::cos(ph), sp = std::sin(ph);
-            const float rh[3] = {st * cp, st * sp, ct};
-            const float thh[3] = {ct * cp, ct * sp, -st};
-            const float phh[3] = {-sp, cp, 0.0f};
-            float Nt[2] = {0, 0}, Np[2] = {0, 0}, Lt[2] = {0, 0}, Lp[2] = {0, 0};
-            for (size_t f = 0; f < nh; ++f)
-            {
-                const Cur &c = cur[f];
-                const float pha = k0 * (rh[0] * c.pos.x + rh[1] * c.pos.y + rh[2] * c.pos.z);
-                const float cr = std::cos(pha), ci = std::sin(pha);
-                auto proj = [](const float *v, const float *u) { return v[0] * u[0] + v[1] * u[1] + v[2] * u[2]; };
-                const float jt[2] = {proj(c.jr, thh), proj(c.ji, thh)};
-                const float jp[2] = {proj(c.jr, phh), proj(c.ji, phh)};
-                const float mt[2] = {proj(c.mr, thh), proj(c.mi, thh)};
-                const float mp[2] = {proj(c.mr, phh), proj(c.mi, phh)};
-                // (a + jb)(cr + j ci)
-                Nt[0] += jt[0] * cr - jt[1] * ci;
-                Nt[1] += jt[0] * ci + jt[1] * cr;
-                Np[0] += jp[0] * cr - jp[1] * ci;
-                Np[1] += jp[0] * ci + jp[1] * cr;
-                Lt[0] += mt[0] * cr - mt[1] * ci;
-                Lt[1] += mt[0] * ci + mt[1] * cr;
-                Lp[0] += mp[0] * cr - mp[1] * ci;
-                Lp[1] += mp[0] * ci + mp[1] * cr;
-            }
-            const float ar = Lp[0] + Nt[0], ai = Lp[1] + Nt[1];
-            const float br = Lt[0] - Np[0], bi = Lt[1] - Np[1];
-            out.U[(size_t)it * nPhi + ip] = ar * ar + ai * ai + br * br + bi * bi;
-        }
-    }
-    float uMax = 0.0f;
-    int iMax = 0;
-    double prad = 0.0;
-    const float dTh = PI / (nTheta - 1), dPh = 2.0f * PI / (nPhi - 1);
-    for (int it = 0; it < nTheta; ++it)
-    {
-        const float th = PI * it / (nTheta - 1);
-        const float wt = (it == 0 || it == nTheta - 1) ? 0.5f : 1.0f;
-        for (int ip = 0; ip < nPhi; ++ip)
-        {
-            const float wp = (ip == 0 || ip == nPhi - 1) ? 0.5f : 1.0f;
-            const float u = out.U[(size_t)it * nPhi + ip];
-            if (u > uMax)
-            {
-                uMax = u;
-                iMax = it * nPhi + ip;
-            }
-            prad += (double)u * std::sin(th) * wt * wp * dTh * dPh;
-        }
-    }
-    out.uMax = uMax;
-    out.directivity = prad > 0 ? (float)(4.0 * M_PI * uMax / prad) : 0.0f;
-    out.peakThetaDeg = 180.0f * (iMax / nPhi) / (nTheta - 1);
-    out.peakPhiDeg = 360.0f * (iMax % nPhi) / (nPhi - 1);
     return true;
 }

@@ -65,10 +65,17 @@ public:
     virtual const FieldProbe &probe(int idx) const = 0;
 };
 
-// Near-to-far-field transform of the Huygens phasors (TlmSolver.h:196-199).
+// The reference's worker pool (TlmSolver.h:34-54). The device transform
+// below takes the reference's signature; it needs no host pool.
+class ThreadPool;
+
+// Near-to-far-field transform of the Huygens phasors, with the reference's
+// signature (TlmSolver.h:196-199). It runs on the HIP device; `pool` is
+// accepted for source compatibility and not used. Without a device it
+// returns false, as RunGpuTlm does.
 bool ComputeFarFieldFromHuygens(const std::vector<HuyFace> &huyFaces,
                                 const std::vector<float> &huyDft, int huySamples, float f0,
-                                FarFieldData &out, int nTheta, int nPhi);
+                                ThreadPool *pool, FarFieldData &out, int nTheta, int nPhi);
 
 class TlmSolver : public IFieldSolver
 {
