@@ -125,6 +125,8 @@ SIGNATURES = {
                                 C.c_int, C.c_int, fp, C.POINTER(FarFieldSummary)]),
     "tlm_far_field_ctx": (C.c_int, [P, fp, C.c_int, C.c_float, C.c_int, C.c_int, fp,
                                     C.POINTER(FarFieldSummary)]),
+    "tlm_far_field_ctx_noalloc": (C.c_int, [P, fp, C.c_int, C.c_float, C.c_int, C.c_int, fp,
+                                    C.POINTER(FarFieldSummary)]),
 }
 
 

@@ -32,6 +32,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <vector>
 
 using namespace tlmi;
@@ -230,6 +232,48 @@ struct DevBufs
     }
 };
 
+// Buffers carved in order from a fixed device workspace (256-byte aligned
+// pieces); nothing is allocated or freed.
+struct Carve
+{
+    unsigned char *base = nullptr;
+    size_t cap = 0, off = 0;
+    static size_t piece(size_t bytes)
+    {
+        return (std::max<size_t>(bytes, 1) + 255) / 256 * 256;
+    }
+    template <class T> hipError_t get(T **x, size_t n)
+    {
+        const size_t b = piece(n * sizeof(T));
+        if (off + b > cap)
+        {
+            *x = nullptr;
+            return hipErrorOutOfMemory;
+        }
+        *x = reinterpret_cast<T *>(base + off);
+        off += b;
+        return hipSuccess;
+    }
+};
+
+// Face ranges of k_ff_sums: enough that the grid fills the chip, each range
+// at least a block of faces.
+int face_ranges(int nf, int ndir)
+{
+    const int ngrp = (ndir + kDirs - 1) / kDirs;
+    return std::max(1, std::min((nf + kFfThreads - 1) / kFfThreads, (2048 + ngrp - 1) / ngrp));
+}
+
+// Workspace of one transform carved by Carve: face records, the direction
+// tables, the partial sums and U.
+size_t far_field_workspace(int nf, int ndir)
+{
+    const size_t nfr = (size_t)face_ranges(nf, ndir);
+    return Carve::piece((size_t)nf * sizeof(FfFace)) + Carve::piece(9 * (size_t)ndir * sizeof(float)) +
+           Carve::piece(3 * (size_t)ndir * sizeof(float)) +
+           Carve::piece((size_t)ndir * nfr * kAcc * sizeof(float)) + Carve::piece((size_t)ndir * sizeof(float));
+}
+
 #define FFCHK(call)                                                                                \
     do                                                                                             \
     {                                                                                              \
@@ -238,8 +282,10 @@ struct DevBufs
             return hip_fail(c, e_, #call);                                                         \
     } while (0)
 
-// Directions, sums, U and the summary for prepared faces (on stream st).
-int far_field_from_faces(tlm_ctx *c, hipStream_t st, DevBufs &b, const FfFace *faces, int nf,
+// Directions, sums, U and the summary for prepared faces (on stream st);
+// device buffers from b (hipMalloc'ed DevBufs or a Carve of a workspace).
+template <class Bufs>
+int far_field_from_faces(tlm_ctx *c, hipStream_t st, Bufs &b, const FfFace *faces, int nf,
                          float f0, int nT, int nP, float *U, tlm_far_field_summary *out)
 {
     const float PI = (float)M_PI;
@@ -264,8 +310,7 @@ int far_field_from_faces(tlm_ctx *c, hipStream_t st, DevBufs &b, const FfFace *f
         }
     }
     const int ngrp = (ndir + kDirs - 1) / kDirs;
-    // enough face ranges that the grid fills the chip, each at least a block
-    const int nfr = std::max(1, std::min((nf + kFfThreads - 1) / kFfThreads, (2048 + ngrp - 1) / ngrp));
+    const int nfr = face_ranges(nf, ndir);
     const int per = (nf + nfr - 1) / nfr;
     float *d_dirs, *d_rh, *d_part, *d_U;
     FFCHK(b.get(&d_dirs, dirs.size()));
@@ -312,6 +357,36 @@ int far_field_from_faces(tlm_ctx *c, hipStream_t st, DevBufs &b, const FfFace *f
         out->peak_theta_deg = 180.0f * (iMax / nP) / (nT - 1);
         out->peak_phi_deg = 360.0f * (iMax % nP) / (nP - 1);
     }
+    return TLM_OK;
+}
+
+// ---- allocation-free variant ----
+// Its workspace: a file-scope device array of kScratchBytes, one instance per
+// GPU the library runs on (held until the process ends). Host code reaches
+// it only through the address hipGetSymbolAddress gives on each device.
+constexpr size_t kScratchBytes = (size_t)128 << 20;
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+unsigned char *g_scratch[kMaxDevices] = {};
+
+// Device address of the workspace on device dev, which must be current;
+// looked up once per device.
+int scratch_on(tlm_ctx *c, int dev, unsigned char **out)
+{
+    if (dev < 0 || dev >= kMaxDevices)
+        return fail(c, TLM_ERR_UNSUPPORTED, "allocation-free far field: device ordinal above 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev])
+    {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return hip_fail(c, e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+        g_scratch[dev] = static_cast<unsigned char *>(p);
+    }
+    *out = g_scratch[dev];
     return TLM_OK;
 }
 
@@ -373,6 +448,35 @@ int tlm_far_field_ctx(tlm_ctx *c, const float origin[3], int samples, float f0, 
         return fail(c, TLM_ERR_STATE, "tlm_far_field_ctx: no Huygens data");
     FFCHK(hipSetDevice(c->dev));
     DevBufs b;
+    FfFace *d_faces;
+    FFCHK(b.get(&d_faces, (size_t)c->nh));
+    k_ff_faces_ctx<<<(c->nh + 255) / 256, 256, 0, c->stream>>>(c->nh, c->hcell, c->haxis, c->huy8,
+                                                                c->nx, c->ny, c->nz, c->Pp, origin[0],
+                                                                origin[1], origin[2], c->dl, d_faces);
+    FFCHK(hipGetLastError());
+    return far_field_from_faces(c, c->stream, b, d_faces, c->nh, f0, n_theta, n_phi, U, out);
+}
+
+int tlm_far_field_ctx_noalloc(tlm_ctx *c, const float origin[3], int samples, float f0, int n_theta,
+                              int n_phi, float *U, tlm_far_field_summary *out)
+{
+    if (!c || !origin || n_theta < 2 || n_phi < 2)
+        return fail(c, TLM_ERR_ARG, "tlm_far_field_ctx_noalloc: bad arguments");
+    if (c->nh == 0 || samples == 0)
+        return fail(c, TLM_ERR_STATE, "tlm_far_field_ctx_noalloc: no Huygens data");
+    const size_t need = far_field_workspace(c->nh, n_theta * n_phi);
+    if (need > kScratchBytes)
+        return fail(c, TLM_ERR_UNSUPPORTED,
+                    "tlm_far_field_ctx_noalloc: " + std::to_string(c->nh) + " faces x " +
+                        std::to_string(n_theta * n_phi) + " directions need " + std::to_string(need) +
+                        " bytes of workspace, above its fixed " + std::to_string(kScratchBytes) +
+                        " (tlm_far_field_ctx allocates what it needs)");
+    FFCHK(hipSetDevice(c->dev));
+    Carve b;
+    int rc = scratch_on(c, c->dev, &b.base);
+    if (rc)
+        return rc;
+    b.cap = kScratchBytes;
     FfFace *d_faces;
     FFCHK(b.get(&d_faces, (size_t)c->nh));
     k_ff_faces_ctx<<<(c->nh + 255) / 256, 256, 0, c->stream>>>(c->nh, c->hcell, c->haxis, c->huy8,

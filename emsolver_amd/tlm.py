@@ -776,13 +776,16 @@ def far_field(pos, axis, sign, dft8, f0, n_theta=37, n_phi=73, samples=1, device
                     sm.peak_phi_deg)
 
 
-def far_field_ctx(dev: TlmDevice, origin, samples, f0, n_theta=37, n_phi=73) -> FarField:
-    """The same from a context's resident Huygens accumulators."""
+def far_field_ctx(dev: TlmDevice, origin, samples, f0, n_theta=37, n_phi=73,
+                  noalloc=False) -> FarField:
+    """The same from a context's resident Huygens accumulators; noalloc=True
+    calls the variant whose workspace is the library's fixed device array
+    (tlm_far_field_ctx_noalloc) instead of allocations of its own."""
     o = np.ascontiguousarray(np.asarray(origin, np.float32).reshape(3))
     U = np.zeros(n_theta * n_phi, np.float32)
     sm = _native.FarFieldSummary()
-    dev._chk(dev.lib.tlm_far_field_ctx(dev.h, ptr(o, C.c_float), int(samples), float(f0),
-                                       n_theta, n_phi, ptr(U, C.c_float), C.byref(sm)),
-             "tlm_far_field_ctx")
+    name = "tlm_far_field_ctx_noalloc" if noalloc else "tlm_far_field_ctx"
+    dev._chk(getattr(dev.lib, name)(dev.h, ptr(o, C.c_float), int(samples), float(f0),
+                                    n_theta, n_phi, ptr(U, C.c_float), C.byref(sm)), name)
     return FarField(U.reshape(n_theta, n_phi), sm.u_max, sm.directivity, sm.peak_theta_deg,
                     sm.peak_phi_deg)

@@ -410,6 +410,19 @@ int tlm_far_field(int device, int nfaces, const float *pos, const int32_t *axis,
 int tlm_far_field_ctx(tlm_ctx *ctx, const float origin[3], int samples, float f0, int n_theta,
                       int n_phi, float *U, tlm_far_field_summary *out);
 
+/* tlm_far_field_ctx without device allocation: no counterpart in the
+ * reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * transform's workspace (64-byte face records, direction tables, partial sums,
+ * U) comes from a fixed 128 MiB device array that the library holds on each
+ * GPU it runs on until the process ends: up to about 2 million Huygens faces
+ * on the default 37 x 73 grid. Larger inputs are refused with
+ * TLM_ERR_UNSUPPORTED. Calls on one GPU share that array: calls that overlap
+ * on one GPU (from different threads or streams) race. Same results as
+ * tlm_far_field_ctx, bit for bit. */
+int tlm_far_field_ctx_noalloc(tlm_ctx *ctx, const float origin[3], int samples, float f0,
+                              int n_theta, int n_phi, float *U, tlm_far_field_summary *out);
+
 #ifdef __cplusplus
 }
 #endif
