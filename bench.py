@@ -3,21 +3,26 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-Workload (one "step" = one TLM time step over the whole grid): the 256^3
-scene with 8 seeded dielectric boxes (eps_r in [1,10], sigma in [0,0.05]
-S/m), a PEC ground plane at z = 0, a centre Ez Gaussian-derivative soft
-source, 16 seeded E probes and a running FP64 probe DFT at 32 frequencies,
-with the PEC surface-current and energy monitors on, as the reference runs
-them. Inputs are resident in HBM before the timed region.
+Workload (one "step" = one TLM time step over the whole grid): the 512^3
+grid with the dielectric layout of the 256^3 configuration scaled up to 64
+seeded boxes (eps_r in [1,10], sigma in [0,0.05] S/m), a PEC ground plane at
+z = 0, a centre Ez Gaussian-derivative soft source, 16 seeded E probes and a
+running FP64 probe DFT at 32 frequencies, with the PEC surface-current and
+energy monitors on, as the reference runs them; link pulses in FP32. Inputs
+are resident in HBM before the timed region.
 
 Prints ONE JSON line. `value` is Gcell-updates/s of the timed K steps
 (wall clock between two stream synchronisations). `roofline` prices the
 fused scatter/connect kernel: algorithmic bytes per launch
 (cells * (97 + 24 * f_diel) B for FP32 pulses) over its average launch
-time, measured with HIP events carried by every 7th step launch of the
-timed region on the context stream, against the 8 TB/s HBM peak. `cpu_baseline` times the CPU
-restatement of the reference algorithm (oracle/, the reference's threading
-rule min(16, hw-1)) on a bounded sample of the same workload.
+time, measured with HIP events carried by the step launches of the timed
+region on the context stream, against the 8 TB/s HBM peak. `cpu_baseline`
+times the CPU restatement of the reference algorithm (oracle/, the
+reference's threading rule min(16, hw-1)) over a 100-step window of the same
+scene. `extra` holds the other configurations: the 256^3 scene, the 512^3
+scene in FP16 / BF16 storage and in free space, the 16-bit storage error
+against the FP64 oracle, the 64^3 free-space case, the batched and served
+workloads, scene set-up and the far-field transform.
 """
 from __future__ import annotations
 
@@ -89,10 +94,11 @@ def read_traffic(workload):
 KERNEL_EVERY = 7
 
 
-def gpu_run(scene, warmup, steps, tag, plain=False):
-    """Time `steps` steps after `warmup`, every KERNEL_EVERY-th step launch
-    carrying HIP events for the kernel average. plain=True times `steps`
-    more steps with no events at all."""
+def gpu_run(scene, warmup, steps, tag, plain=False, kernel_every=None):
+    """Time `steps` steps after `warmup`, every kernel_every-th step launch
+    (default KERNEL_EVERY) carrying HIP events for the kernel average.
+    plain=True times `steps` more steps with no events at all."""
+    kernel_every = KERNEL_EVERY if kernel_every is None else kernel_every
     from emsolver_amd import tlm
     sc = scene.with_steps(warmup + steps * (2 if plain else 1))
     t_setup = time.perf_counter()
@@ -102,7 +108,7 @@ def gpu_run(scene, warmup, steps, tag, plain=False):
         dev.run(0, warmup)
     dev.sync()
     t0 = time.perf_counter()
-    _, ms_kernel = dev.time_steps(warmup, steps, kernel_every=KERNEL_EVERY)
+    _, ms_kernel = dev.time_steps(warmup, steps, kernel_every=kernel_every)
     dev.sync()
     wall = time.perf_counter() - t0
     cells = sc.ncell
@@ -111,7 +117,8 @@ def gpu_run(scene, warmup, steps, tag, plain=False):
         "gcells": cells * steps / wall / 1e9,
         "ms_per_step": wall * 1e3 / steps,
         "kernel_ms_avg": ms_kernel,
-        "kernel_launches_timed": len(range(0, steps, KERNEL_EVERY)),
+        "kernel_launches_timed": len(range(0, steps, kernel_every)),
+        "kernel_every": kernel_every,
     }
     if plain:
         t1 = time.perf_counter()
@@ -454,13 +461,15 @@ def free_space_64_run(with_cpu=True):
     return out
 
 
-def cpu_baseline(scene, budget_s=15.0):
+def cpu_baseline(scene, budget_s=45.0, window=100):
+    """The oracle on the same scene: a fixed window of `window` steps after
+    two warm-up steps (fewer if the window would exceed budget_s)."""
     from tests import helpers
     o = helpers.oracle_open(scene.with_steps(10**6))
     t0 = time.perf_counter()
     o.run_steps(2)
     per = (time.perf_counter() - t0) / 2
-    n = int(max(2, min(200, budget_s / max(per, 1e-6))))
+    n = int(max(2, min(window, budget_s / max(per, 1e-6))))
     t0 = time.perf_counter()
     o.run_steps(n)
     dt = time.perf_counter() - t0
@@ -493,17 +502,24 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--size", type=int, default=256, help="grid edge of the dielectric-box scene")
+    ap.add_argument("--size", type=int, default=512,
+                    help="grid edge of the dielectric-box scene (64 boxes at 512, else 8)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-extra", action="store_true", help="skip the 512^3 free-space line")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the extra single-grid lines (256^3, 16-bit, free space, precision, "
+                         "set-up, far field)")
+    ap.add_argument("--cpu-budget", type=float, default=45.0,
+                    help="cap on the CPU baseline's 100-step window, seconds")
+    ap.add_argument("--precision-steps", type=int, default=4000,
+                    help="steps of the 16-bit error study (the workload's 4000)")
     ap.add_argument("--extra-steps", type=int, default=200,
                     help="timed steps of each extra 512^3 line")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched and served lines")
     ap.add_argument("--lines", default="",
-                    help="comma list of extra lines to run (free_space_64, diel_boxes_512_f32, diel_boxes_512_f16, "
-                         "diel_boxes_512_bf16, free_space_512_f32, scene_setup_256, "
-                         "scene_setup_512, ntff_256, batched_256, served); default all")
+                    help="comma list of extra lines to run (diel_boxes_256, diel_boxes_512_f16, "
+                         "diel_boxes_512_bf16, free_space_512_f32, precision, free_space_64, "
+                         "scene_setup_256, scene_setup_512, ntff_256, batched_256, served); "
+                         "default all")
     ap.add_argument("--no-graph", action="store_true",
                     help="plain launches instead of captured HIP graphs in the batched lines "
                          "(for tracing tools that do not follow graph launches)")
@@ -534,7 +550,7 @@ def main():
     if args.no_graph:
         GRAPH_TICKS = 0
     if args.precision_report:
-        print(json.dumps(precision_report()), flush=True)
+        print(json.dumps(precision_report(steps=args.precision_steps)), flush=True)
         return
     if args.serve_only:
         print(json.dumps(served_run(args.serve_count, rate=args.serve_rate,
@@ -545,12 +561,15 @@ def main():
     log(f"[bench] device {dev_name}")
 
     n = args.size
+    nboxes = 64 if n >= 512 else 8
     workload = f"diel_boxes_{n}"
-    scene = scenes.dielectric_boxes(n=n, nboxes=8, steps=args.warmup + args.steps, nprobes=16,
+    scene = scenes.dielectric_boxes(n=n, nboxes=nboxes, steps=args.warmup + args.steps, nprobes=16,
                                     nfreq=32, seed=1)
-    log(f"[bench] {workload}: {scene.ncell} cells, f_diel {scene.diel_fraction():.4f}, "
-        f"warmup {args.warmup}, steps {args.steps}")
-    main_run = gpu_run(scene, args.warmup, args.steps, workload, plain=True)
+    log(f"[bench] {workload}: {scene.ncell} cells, {nboxes} boxes, f_diel "
+        f"{scene.diel_fraction():.4f}, warmup {args.warmup}, steps {args.steps}")
+    # at 512^3 a launch takes ~2.7 ms: events on every launch cost nothing there
+    main_run = gpu_run(scene, args.warmup, args.steps, workload, plain=True,
+                       kernel_every=1 if scene.ncell >= 512 ** 3 else KERNEL_EVERY)
     log(f"[bench] {workload}: {main_run['gcells']:.2f} Gcell/s wall, kernel "
         f"{main_run['kernel_ms_avg']:.3f} ms, {main_run['achieved_gbs']:.0f} GB/s algorithmic; "
         f"{main_run['plain_gcells']:.2f} Gcell/s without launch events")
@@ -559,8 +578,8 @@ def main():
         "gcell_s": round(main_run["plain_gcells"], 3),
         "steps": args.steps,
         "note": f"the next steps of the same run with no HIP events on the launches; in the "
-                f"timed steps every {KERNEL_EVERY}th step launch carries a start and an end "
-                f"event for the kernel average"}}
+                f"timed steps every {main_run['kernel_every']}th step launch carries a start "
+                f"and an end event for the kernel average"}}
     lib = tlm._native.load()
     import ctypes as C
     gbps, msc = C.c_float(0), C.c_float(0)
@@ -572,37 +591,49 @@ def main():
     def want(name):
         return lines is None or name in lines
 
+    def roofline_line(r, sc):
+        return {
+            "gcell_s": round(r["gcells"], 3),
+            "kernel_ms_avg": round(r["kernel_ms_avg"], 4),
+            "kernel_launches_timed": r["kernel_launches_timed"],
+            "achieved_gbs": round(r["achieved_gbs"], 1),
+            "roofline_frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
+            "alg_bytes_per_cell": round(r["alg_bytes_per_cell"], 3),
+            "diel_fraction": round(sc.diel_fraction(), 5),
+        }
+
+    if not args.no_extra and want("diel_boxes_256") and n != 256:
+        sc = scenes.dielectric_boxes(n=256, nboxes=8, steps=args.extra_steps + 10, nprobes=16,
+                                     nfreq=32, seed=1)
+        r = gpu_run(sc, 10, args.extra_steps, "diel_boxes_256")
+        extra["diel_boxes_256"] = roofline_line(r, sc)
+        extra["diel_boxes_256"]["steps"] = args.extra_steps
+        extra["diel_boxes_256"]["traffic"] = read_traffic("diel_boxes_256")
+        log(f"[bench] diel_boxes_256: {r['gcells']:.2f} Gcell/s, kernel {r['kernel_ms_avg']:.3f} ms, "
+            f"{r['achieved_gbs']:.0f} GB/s = {r['achieved_gbs'] / HBM_PEAK_GBS:.3f} of peak")
     if not args.no_extra:
-        for pulse in [p for p in ("f32", "f16", "bf16") if want(f"diel_boxes_512_{p}")]:
+        for pulse in [p for p in ("f16", "bf16") if want(f"diel_boxes_512_{p}")]:
             sc16 = scenes.dielectric_boxes(n=512, nboxes=64, steps=args.extra_steps + 10,
                                            nprobes=16, nfreq=32, seed=1)
             sc16.pulse = pulse
-            r = gpu_run(sc16, 10, args.extra_steps, f"diel_boxes_512_{pulse}")
-            extra[f"diel_boxes_512_{pulse}"] = {
-                "gcell_s": round(r["gcells"], 3),
-                "kernel_ms_avg": round(r["kernel_ms_avg"], 4),
-                "achieved_gbs": round(r["achieved_gbs"], 1),
-                "roofline_frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                "alg_bytes_per_cell": round(r["alg_bytes_per_cell"], 3),
+            r = gpu_run(sc16, 10, args.extra_steps, f"diel_boxes_512_{pulse}", kernel_every=1)
+            extra[f"diel_boxes_512_{pulse}"] = roofline_line(r, sc16)
+            extra[f"diel_boxes_512_{pulse}"].update({
                 "fp32_slab_fraction": round(fp32_slab_fraction(sc16), 4),
                 "steps": args.extra_steps,
-            }
+                "traffic": read_traffic(f"diel_boxes_512_{pulse}")})
             log(f"[bench] diel_boxes_512_{pulse}: {r['gcells']:.2f} Gcell/s, kernel "
                 f"{r['kernel_ms_avg']:.3f} ms, {r['achieved_gbs']:.0f} GB/s = "
                 f"{r['achieved_gbs'] / HBM_PEAK_GBS:.3f} of peak")
     if not args.no_extra and want("free_space_512_f32"):
         big = scenes.free_space_cube(n=512, steps=args.extra_steps + 10, probe_offset=10)
-        r = gpu_run(big, 10, args.extra_steps, "free_space_512")
-        extra["free_space_512_f32"] = {
-            "gcell_s": round(r["gcells"], 3),
-            "kernel_ms_avg": round(r["kernel_ms_avg"], 4),
-            "achieved_gbs": round(r["achieved_gbs"], 1),
-            "roofline_frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4),
-            "alg_bytes_per_cell": r["alg_bytes_per_cell"],
-            "steps": args.extra_steps,
-        }
+        r = gpu_run(big, 10, args.extra_steps, "free_space_512", kernel_every=1)
+        extra["free_space_512_f32"] = roofline_line(r, big)
+        extra["free_space_512_f32"]["steps"] = args.extra_steps
         log(f"[bench] free_space_512: {r['gcells']:.2f} Gcell/s, kernel {r['kernel_ms_avg']:.3f} ms, "
             f"{r['achieved_gbs']:.0f} GB/s = {r['achieved_gbs'] / HBM_PEAK_GBS:.3f} of peak")
+    if not args.no_extra and want("precision"):
+        extra["precision_512_crop128"] = precision_report(steps=args.precision_steps)
 
     if not args.no_extra and want("free_space_64"):
         extra["free_space_64"] = free_space_64_run(with_cpu=not args.no_cpu)
@@ -638,10 +669,10 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"{workload}: {n}^3 SCN TLM, 8 seeded dielectric boxes "
+            "workload": f"{workload}: {n}^3 SCN TLM, {nboxes} seeded dielectric boxes "
                         "(eps_r U[1,10], sigma U[0,0.05] S/m), PEC ground plane z=0, centre Ez "
                         "Gaussian-derivative soft source (w=20, delay 60 steps), 16 E probes, "
-                        "FP64 probe DFT at 32 frequencies, Js + energy monitors",
+                        "FP64 probe DFT at 32 frequencies, Js + energy monitors, FP32 pulses",
             "grid": [n, n, n],
             "diel_fraction": round(scene.diel_fraction(), 5),
             "device": dev_name,

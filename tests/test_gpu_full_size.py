@@ -2,8 +2,8 @@
 
 * The 256^3 dielectric-box scene (16.8 M cells, 16 probes with a
   32-frequency FP64 DFT, the ground plane's Js faces, energy) and the 512^3
-  64-box scene (134 M cells; FP32 and FP16 storage) against the oracle bit
-  for bit over their first steps, at the sizes where offsets, launch grids
+  64-box scene (134 M cells; FP32, FP16 and BF16 storage) against the oracle
+  bit for bit over their first steps, at the sizes where offsets, launch grids
   and the XCD block ranges take their full-scale values.
 * The batched configuration's 256 grids and requests of the served workload
   at its full range of sizes: the same bits in the mix as alone.
@@ -56,10 +56,14 @@ def test_512_scene_bitexact_first_steps_fp32():
     _check(_near_probes(sc, [(2, 0, 0), (0, 3, 0), (-1, -2, 1)]))
 
 
-def test_512_scene_bitexact_first_steps_fp16():
+@pytest.mark.parametrize("pulse", ["f16", "bf16"])
+def test_512_scene_bitexact_first_steps_16bit(pulse):
+    """The 512^3 64-box scene in both 16-bit storage formats (the two runs of
+    the mixed-precision configuration) against the oracle's restatement of
+    the 16-bit scheme, bit for bit."""
     sc = scenes.dielectric_boxes(n=512, nboxes=64, steps=6, nprobes=16, nfreq=4, seed=1)
     _check16(dataclasses.replace(_near_probes(sc, [(2, 0, 0), (0, 3, 0), (-1, -2, 1)]),
-                                 pulse="f16"))
+                                 pulse=pulse))
 
 
 def _weighted_energy(sc, V, Vs):
