@@ -203,11 +203,37 @@ def batched_run(count=256, sample_every=4):
     return out
 
 
+def scene_alg_bytes(sc):
+    """B_alg per cell-update of a scene (host map or device set-up
+    operations): 2*12*s_p + 1 + 24*f_diel, s_p counted per slab for the
+    16-bit formats (4 B in the FP32 band round the feed or sources)."""
+    from emsolver_amd import scenes
+    if sc.mat is not None:
+        f_diel = sc.diel_fraction()
+    else:
+        m = np.zeros((sc.nz, sc.ny, sc.nx), np.uint8)
+        for op in sc.ops:
+            if op[0] == "cells_box":
+                (i0, j0, k0), (i1, j1, k1), mid = op[1], op[2], op[3]
+                m[k0:k1, j0:j1, i0:i1] = mid
+        f_diel = float((m >= scenes.MAT_DIEL0).mean())
+    sp = PULSE_BYTES[sc.pulse]
+    if sc.pulse != "f32":
+        if sc.dipole is not None and not sc.sources and sc.fp32_halo >= 0:
+            _, _, g0, gap = sc.dipole
+            lo, hi = max(0, g0 - sc.fp32_halo), min(sc.nz - 1, g0 + gap - 1 + sc.fp32_halo)
+            sp = 2 + 2 * (hi - lo + 1) / sc.nz
+        else:
+            sp = 2 + 2 * fp32_slab_fraction(sc)
+    return 2 * 12 * sp + 1 + 2 * 3 * 4 * f_diel
+
+
 def served_run(count=200, rate=50.0, baseline_sample=20, seed=5, max_cells=int(2.5e8),
-               device_setup=False):
+               device_setup=False, copy_gbs=None):
     """The served mixed-size stream (Poisson arrivals at `rate`/s, axes
-    log-uniform in [32,384], steps U[500,4000], FP16 storage for the requests
-    that tolerate 1e-3, FP32 otherwise) through the resident serving loop.
+    log-uniform in [32,384], steps U[500,4000]; requests that tolerate 1e-3
+    in FP16 where the storage policy's error model allows it, all others in
+    FP32) through the resident serving loop.
     device_setup: each request's scene is voxelised on the device at
     admission instead of built and uploaded from the host. Baseline:
     `baseline_sample` of the same requests solved one at a time as plain FP32
@@ -240,11 +266,40 @@ def served_run(count=200, rate=50.0, baseline_sample=20, seed=5, max_cells=int(2
     summ["one_at_a_time_fp32_gcell_s"] = round(alone_work / alone_wall / 1e9, 3)
     summ["one_at_a_time_sample"] = f"first {baseline_sample} requests, FP32, alone"
     summ["speedup_vs_one_at_a_time"] = round(summ["gcell_s"] / summ["one_at_a_time_fp32_gcell_s"], 3)
+    # the HBM roofline of this mix: every request's cell-updates at its own
+    # algorithmic bytes (FP16 requests: 16-bit slabs outside the FP32 band)
+    # moved at 8 TB/s; no schedule of the same requests can beat it
+    work = byts = f16_work = 0.0
+    for r in reqs:
+        sc = r.build()
+        w = sc.ncell * sc.total_steps
+        work += w
+        byts += w * scene_alg_bytes(sc)
+        f16_work += w if sc.pulse != "f32" else 0.0
+    ceil = work / (byts / (HBM_PEAK_GBS * 1e9)) / 1e9
+    summ["hbm_ceiling_gcell_s"] = round(ceil, 3)
+    summ["mix_alg_bytes_per_cell"] = round(byts / work, 3)
+    summ["ceiling_vs_one_at_a_time"] = round(ceil / summ["one_at_a_time_fp32_gcell_s"], 3)
+    summ["frac_of_ceiling"] = round(summ["gcell_s"] / ceil, 4)
+    if copy_gbs:
+        # the same at the bandwidth a plain device copy reaches on this box
+        cc = work / (byts / (copy_gbs * 1e9)) / 1e9
+        summ["copy_ceiling_gcell_s"] = round(cc, 3)
+        summ["frac_of_copy_ceiling"] = round(summ["gcell_s"] / cc, 4)
+    summ["target_vs_one_at_a_time"] = 5.0
+    summ["f16_share"] = round(summ["f16_requests"] / summ["requests"], 4)
+    summ["f16_work_share"] = round(f16_work / work, 4)
+    summ["storage_policy"] = (f"tolerance 1e-3 -> FP16 only where serve.plan_storage's error model "
+                              f"allows it (FP32 band holds the dipole and probes with "
+                              f"{serve.F16_MARGIN} slabs to spare, axes <= {serve.F16_MAX_AXIS}, "
+                              f">= {serve.F16_MIN_SHARE} of the slabs 16-bit), else FP32")
     for k in ("p50_latency_s", "p99_latency_s", "mean_latency_s", "gcell_s", "makespan_s"):
         summ[k] = round(summ[k], 4)
     log(f"[bench] served_{count}: p50 {summ['p50_latency_s']:.2f}s p99 {summ['p99_latency_s']:.2f}s, "
         f"{summ['gcell_s']:.2f} Gcell/s aggregate (mean {summ['mean_members']} members); "
-        f"one at a time {summ['one_at_a_time_fp32_gcell_s']:.2f} Gcell/s")
+        f"one at a time {summ['one_at_a_time_fp32_gcell_s']:.2f} Gcell/s; HBM ceiling of the mix "
+        f"{ceil:.1f} Gcell/s = {summ['ceiling_vs_one_at_a_time']:.2f}x one at a time (target 5x), "
+        f"reached {summ['frac_of_ceiling']:.3f}; FP16 {summ['f16_share']:.3f} of requests")
     return summ
 
 
@@ -530,8 +585,8 @@ def main():
     ap.add_argument("--serve-host-setup", action="store_true",
                     help="served requests as host material maps uploaded at admission (default: "
                          "set-up operations voxelised on the device)")
-    ap.add_argument("--serve-count", type=int, default=200,
-                    help="requests of the served line (the full stream is 2000)")
+    ap.add_argument("--serve-count", type=int, default=2000,
+                    help="requests of the served line (the workload's full stream: 2000)")
     ap.add_argument("--serve-rate", type=float, default=50.0,
                     help="Poisson arrival rate of the served line, requests/s (the workload's is 50)")
     args = ap.parse_args()
@@ -646,7 +701,8 @@ def main():
         extra["batched_256"] = batched_run(256)
     if not args.no_batch and want("served"):
         extra[f"served_{args.serve_count}"] = served_run(args.serve_count, rate=args.serve_rate,
-                                                         device_setup=not args.serve_host_setup)
+                                                         device_setup=not args.serve_host_setup,
+                                                         copy_gbs=extra.get("device_copy_gbs"))
 
     cpu = None
     if not args.no_cpu:

@@ -79,6 +79,7 @@ class Scene:
     Vs0: np.ndarray | None = None
     name: str = "scene"
     ops: list | None = None       # device set-up operations (mat is None then)
+    dipole: tuple | None = None   # dipole_scene: (k0, k1, g0, gap) of the wire column and feed gap
 
     @property
     def ncell(self) -> int:
@@ -209,7 +210,8 @@ def dipole_scene(nx, ny, nz, length_frac=0.5, steps=1000, background_boxes=0, se
     k0 = max(1, ck - half)
     k1 = min(nz - 2, ck + half)
     g0 = ck - (gap - 1) // 2
-    s = Scene(nx, ny, nz, None, epsr, sigma, total_steps=steps, name=f"dipole_{nx}x{ny}x{nz}")
+    s = Scene(nx, ny, nz, None, epsr, sigma, total_steps=steps, name=f"dipole_{nx}x{ny}x{nz}",
+              dipole=(k0, k1, g0, gap))
     if device_setup:
         if gap > 6:
             raise ValueError("a feed carve reaches every gap cell for gaps of up to 6 cells")

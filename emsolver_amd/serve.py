@@ -50,9 +50,42 @@ class Completion:
         return self.finished - self.arrival
 
 
-def storage_for(tol: float) -> str:
-    """FP16 pulse storage where the client tolerates 1e-3, else FP32."""
-    return "f16" if tol >= 1e-3 else "f32"
+# Storage policy: the error model of FP16 pulse storage for a request that
+# tolerates 1e-3 of probe error against the FP64 oracle (max over the run of
+# |x - x64| / max |x64| per probe). Measured with the oracle's bit-exact
+# restatement of the 16-bit scheme on served-stream requests (dipole over
+# 8 dielectric boxes, 500-4000 steps):
+#   * a probe read from an FP16 slab: 1.3e-3 to 1.8e-3 (fails at any size);
+#   * probes and the whole dipole inside the FP32 band: <= 5.5e-4 for grids
+#     up to 80 cells per axis (probes >= 2 slabs inside the band, FP16 slab
+#     shares up to 0.52), 7.8e-4 at 99 x 135 x 122 (share 0.25); the error
+#     grows with the grid's extent.
+# So a 1e-3 request is stored in FP16 only if its FP32 band (source slab +-
+# halo) holds the dipole and every probe with F16_MARGIN slabs to spare, no
+# axis exceeds F16_MAX_AXIS (the sizes the model was checked on), and at
+# least F16_MIN_SHARE of the slabs stay 16-bit (below that the 16-bit step,
+# slower per byte than the FP32 one, would not pay). Everything else is FP32.
+F16_TOL = 1e-3
+F16_MARGIN = 4
+F16_MAX_AXIS = 112
+F16_MIN_SHARE = 0.4
+
+
+def plan_storage(tol: float, nz: int, dims, src_slabs, keep_slabs):
+    """(pulse, fp32_halo, fp16_slab_share) of a request: src_slabs are the z
+    slabs of its soft sources and ports (the FP32 band is their range +-
+    halo), keep_slabs the slabs that must lie inside the band (its PEC
+    structure, its probes)."""
+    if tol < F16_TOL or not src_slabs:
+        return "f32", 8, 0.0
+    lo_s, hi_s = min(src_slabs), max(src_slabs)
+    reach = max([0] + [lo_s - k for k in keep_slabs] + [k - hi_s for k in keep_slabs])
+    halo = reach + F16_MARGIN
+    band = min(nz - 1, hi_s + halo) - max(0, lo_s - halo) + 1
+    share = 1.0 - band / nz
+    if max(dims) > F16_MAX_AXIS or share < F16_MIN_SHARE:
+        return "f32", 8, share
+    return "f16", halo, share
 
 
 class TlmServer:
@@ -234,14 +267,15 @@ def summarize(done, wall_s=None) -> dict:
 
 
 def served_stream(count=2000, rate=50.0, lo=32, hi=384, steps=(500, 4000), seed=5,
-                  f16_share=0.5, nprobes=2, device_setup=False):
+                  f16_share=0.5, nprobes=2, device_setup=False, policy=True):
     """The served workload: Poisson arrivals at `rate`/s, each axis log-uniform
     in [lo, hi], the dielectric-box background with a centre-fed dipole of
-    seeded length, steps uniform in [steps], per-request tolerance 1e-3
-    (FP16 storage) or 1e-5 (FP32) with a seeded split. Scenes are built on
-    demand (Request.build): as host material maps, or with device_setup as
-    set-up operations that the device voxelises at admission (the same
-    grids)."""
+    seeded length, steps uniform in [steps], per-request tolerance 1e-3 or
+    1e-5 with a seeded split; storage by plan_storage (policy=False: FP16
+    with the default FP32 halo for every 1e-3 request, for tests of the
+    batch with both formats). Scenes are built on demand (Request.build): as
+    host material maps, or with device_setup as set-up operations that the
+    device voxelises at admission (the same grids)."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -254,20 +288,23 @@ def served_stream(count=2000, rate=50.0, lo=32, hi=384, steps=(500, 4000), seed=
         sseed = int(rng.integers(1 << 31))
 
         def build(dims=dims, frac=frac, st=st, tol=tol, sseed=sseed, rid=rid):
-            return request_scene(dims, frac, st, storage_for(tol), sseed, nprobes, f"req{rid}",
-                                 device_setup)
+            return request_scene(dims, frac, st, tol, sseed, nprobes, f"req{rid}", device_setup,
+                                 policy)
 
         out.append(Request(rid, None, t, tol, build))
     return out
 
 
-def request_scene(dims, length_frac, steps, pulse, seed, nprobes=2, name="req",
-                  device_setup=False):
+def request_scene(dims, length_frac, steps, tol, seed, nprobes=2, name="req",
+                  device_setup=False, policy=True):
+    """One served request: the centre-fed dipole over 8 dielectric boxes,
+    `nprobes` seeded E probes off the dipole, stored as plan_storage decides
+    for the tolerance `tol` (policy=False: FP16 with the default FP32 halo
+    whenever tol allows 1e-3)."""
     nx, ny, nz = dims
     sc = scenes.dipole_scene(nx, ny, nz, length_frac=length_frac, steps=steps,
                              background_boxes=8, seed=seed, device_setup=device_setup)
     sc.faces = False
-    sc.pulse = pulse
     rng = np.random.default_rng(seed + 1)
     probes = []
     while len(probes) < nprobes:
@@ -279,4 +316,12 @@ def request_scene(dims, length_frac, steps, pulse, seed, nprobes=2, name="req",
             probes.append(c)
     sc.probes = np.array(probes, np.int64)
     sc.name = name
+    # the band: the feed gap's slabs (the port), holding the wire column and
+    # the probes (a device-built scene has no host map to read them from)
+    k0, k1, g0, gap = sc.dipole
+    keep = [k0, k1] + [int(c) // (nx * ny) for c in sc.probes]
+    if policy:
+        sc.pulse, sc.fp32_halo, _ = plan_storage(tol, nz, dims, list(range(g0, g0 + gap)), keep)
+    else:
+        sc.pulse = "f16" if tol >= F16_TOL else "f32"
     return sc

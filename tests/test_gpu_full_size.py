@@ -139,10 +139,10 @@ def test_batched_256_grids_equal_alone():
 
 def test_served_stream_full_sizes_equal_alone():
     """Twelve requests of the served workload (each axis log-uniform in
-    [32, 384], FP16 or FP32 storage, steps cut to U[40, 120]) all arriving at
-    once: served together, each one's probes and ports equal the same request
-    run alone."""
-    reqs = serve.served_stream(count=12, rate=1e9, steps=(40, 120), seed=5)
+    [32, 384], FP16 for the 1e-3 requests and FP32 for the others, steps cut
+    to U[40, 120]) all arriving at once: served together, each one's probes
+    and ports equal the same request run alone."""
+    reqs = serve.served_stream(count=12, rate=1e9, steps=(40, 120), seed=5, policy=False)
     for r in reqs:
         r.arrival = 0.0
     srv = serve.TlmServer(block_ticks=16)

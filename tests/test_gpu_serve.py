@@ -11,7 +11,9 @@ pytestmark = pytest.mark.gpu
 
 
 def _stream(n=10):
-    reqs = serve.served_stream(count=n, rate=1e9, lo=12, hi=40, steps=(30, 120), seed=21)
+    # FP16 for every 1e-3 request (policy off): the mix holds both formats
+    reqs = serve.served_stream(count=n, rate=1e9, lo=12, hi=40, steps=(30, 120), seed=21,
+                               policy=False)
     for r in reqs:
         r.arrival = 0.0
     return reqs
@@ -57,7 +59,7 @@ def test_served_device_setup_equals_host_setup():
     same requests uploaded as host maps."""
     def stream(dev_setup):
         reqs = serve.served_stream(count=6, rate=1e9, lo=12, hi=36, steps=(30, 90), seed=33,
-                                   device_setup=dev_setup)
+                                   device_setup=dev_setup, policy=False)
         for r in reqs:
             r.arrival = 0.0
         return reqs

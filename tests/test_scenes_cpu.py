@@ -25,8 +25,9 @@ def test_dipole_ops_equal_host_dipole(dims, frac, boxes, seed):
 
 def test_request_scene_host_and_ops_agree():
     for dims, st, pulse, sd in (((40, 35, 50), 300, "f16", 3), ((33, 60, 41), 500, "f32", 8)):
-        a = serve.request_scene(dims, 0.5, st, pulse, sd)
-        b = serve.request_scene(dims, 0.5, st, pulse, sd, device_setup=True)
+        tol = 1e-3 if pulse == "f16" else 1e-5
+        a = serve.request_scene(dims, 0.5, st, tol, sd, policy=False)
+        b = serve.request_scene(dims, 0.5, st, tol, sd, device_setup=True, policy=False)
         assert np.array_equal(a.probes, b.probes)
         rb = helpers.oracle_build(b)
         assert np.array_equal(rb.mat, a.mat) and rb.ports == [(list(a.ports[0][0]), 2, 1.0)]
@@ -71,3 +72,41 @@ def test_plane_wave_sources_are_the_air_cells_of_the_plane():
         assert n == len(want) > 0
         assert [c for c, _, _ in sc.sources] == want
         assert all(p == (axis + 1) % 3 and a == 0.5 for _, p, a in sc.sources)
+
+
+def test_storage_policy_rules():
+    """serve.plan_storage: FP32 below the 1e-3 tolerance; otherwise an FP32
+    band of the source slabs +- halo that holds every slab to keep with
+    F16_MARGIN to spare, FP16 only when the grid is within F16_MAX_AXIS and
+    at least F16_MIN_SHARE of the slabs stay 16-bit."""
+    assert serve.plan_storage(1e-5, 100, (100, 100, 100), [50], [50])[0] == "f32"
+    pulse, halo, share = serve.plan_storage(1e-3, 100, (100, 100, 100), [50], [45, 55, 48])
+    assert pulse == "f16" and halo == 5 + serve.F16_MARGIN
+    assert abs(share - (1 - (2 * halo + 1) / 100)) < 1e-12
+    # a probe far from the source widens the band until FP16 no longer pays
+    pulse, halo, share = serve.plan_storage(1e-3, 100, (100, 100, 100), [50], [45, 55, 90])
+    assert pulse == "f32" and share < serve.F16_MIN_SHARE
+    # beyond the sizes the error model was checked on
+    big = serve.F16_MAX_AXIS + 1
+    assert serve.plan_storage(1e-3, 100, (big, 100, 100), [50], [50])[0] == "f32"
+
+
+def test_served_storage_same_for_host_and_device_setup():
+    """A request gets the same storage and FP32 band whether its scene is a
+    host map or set-up operations; the stream holds FP16 requests only where
+    the policy allows, FP32 otherwise."""
+    host = serve.served_stream(count=120, lo=32, hi=96, seed=5)
+    dev = serve.served_stream(count=120, lo=32, hi=96, seed=5, device_setup=True)
+    n16 = 0
+    for a, b in zip(host, dev):
+        sa, sb = a.build(), b.build()
+        assert (sa.pulse, sa.fp32_halo) == (sb.pulse, sb.fp32_halo)
+        if sa.pulse == "f16":
+            n16 += 1
+            assert a.tol >= serve.F16_TOL
+            assert max(sa.nx, sa.ny, sa.nz) <= serve.F16_MAX_AXIS
+            k0, k1, g0, gap = sa.dipole
+            slab = sa.nx * sa.ny
+            for k in [k0, k1] + [int(c) // slab for c in sa.probes]:
+                assert g0 - sa.fp32_halo + serve.F16_MARGIN <= k <= g0 + gap - 1 + sa.fp32_halo - serve.F16_MARGIN
+    assert n16 >= 1
