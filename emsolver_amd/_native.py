@@ -44,7 +44,8 @@ class FarFieldSummary(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("kernel", C.c_int32), ("planes", C.c_int32), ("block_waves", C.c_int32),
-                ("nontemporal", C.c_int32), ("grid_blocks", C.c_int32), ("fold", C.c_int32)]
+                ("nontemporal", C.c_int32), ("grid_blocks", C.c_int32), ("fold", C.c_int32),
+                ("occupancy", C.c_int32)]
 
 
 # Every symbol declared in include/GpuTlmHip.h with its ctypes signature.

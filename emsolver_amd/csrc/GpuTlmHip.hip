@@ -77,6 +77,30 @@ size_t energy_parts(const tlm_ctx *c)
 // own start / end timestamps (hipExtLaunchKernel): a marker recorded on the
 // stream between two kernels would add a gap of ~5 us before and after the
 // step, which hipEventRecord on either side of it measured.
+// Dynamic LDS per workgroup that holds the step kernel to `occ` waves per
+// SIMD (workgroups of wpb waves, 160 KB of LDS per CU): the kernel uses no
+// LDS, the share only limits how many of its workgroups a CU takes. On
+// large FP32 grids 5 waves per SIMD keep HBM as busy as more do and contend
+// less (512^3 steps: plain air 2.219 against 2.275 ms, 64 dielectric boxes
+// 2.684 against 2.745 ms); at 256^3 a cap costs 1 % and 16-bit steps gain
+// nothing, so those run uncapped.
+constexpr int kLdsPerCu = 160 * 1024;
+int lds_share(int occ, int wpb)
+{
+    if (occ <= 0)
+        return 0;
+    const int wg = std::max(1, occ * 4 / wpb); // workgroups per CU
+    return (kLdsPerCu / wg) & ~1023;           // (wg + 1) shares no longer fit
+}
+
+constexpr long long kOccCapCells = 32LL << 20;
+int step_occupancy(const tlm_ctx *c)
+{
+    if (c->tune.occupancy >= 0)
+        return c->tune.occupancy;
+    return c->pf == TLM_PULSE_F32 && (long long)c->nx * c->ny * c->nz >= kOccCapCells ? 5 : 0;
+}
+
 template <bool S, bool E, int PF>
 void launch_fused(tlm_ctx *c, const tlm::StepArgs &a, const tlm::MonitorArgs &m, int nmon,
                   hipEvent_t e0, hipEvent_t e1)
@@ -84,14 +108,15 @@ void launch_fused(tlm_ctx *c, const tlm::StepArgs &a, const tlm::MonitorArgs &m,
     const int bt = c->wpb * tlm::kWave;
     const int mon_blocks = nmon > 0 ? (nmon + bt - 1) / bt : 0;
     dim3 grid(c->blocks + mon_blocks), block(bt);
+    const int shm = lds_share(step_occupancy(c), c->wpb);
     // automatic: non-temporal pulse stores when the grid has stub cells
     // (faster there by 1-3 %; slower by 3-4 % in plain air)
     const bool nt = c->tune.nontemporal == 1 || (c->tune.nontemporal == 2 && c->stubs);
     auto go = [&](auto kern) {
         if (e0)
-            hipExtLaunchKernelGGL(kern, grid, block, 0, c->stream, e0, e1, 0, a, m, c->blocks);
+            hipExtLaunchKernelGGL(kern, grid, block, shm, c->stream, e0, e1, 0, a, m, c->blocks);
         else
-            hipLaunchKernelGGL(kern, grid, block, 0, c->stream, a, m, c->blocks);
+            hipLaunchKernelGGL(kern, grid, block, shm, c->stream, a, m, c->blocks);
     };
     if (PF != tlm::PF_F32)
         go(tlm::k_step<S, E, false, PF>);
@@ -183,6 +208,10 @@ int launch_step(tlm_ctx *c, int n, const tlm::MonitorArgs &m, int nmon, hipEvent
     a.tasks = c->tasks;
     a.groups = c->groups;
     a.rho = c->rho;
+    a.nmat = c->nmat;
+    a.mstrips = tlm::magic_div((unsigned)c->strips);
+    a.mnx = tlm::magic_div((unsigned)c->nx);
+    a.xm = tlm::xcd_map(c->blocks, c->groups);
     a.fold = fold_of(c, n);
     if (c->pf == TLM_PULSE_F16)
         launch_pf<tlm::PF_F16>(c, a, energy, m, nmon, e0, e1);
@@ -758,17 +787,22 @@ int tlm_upload(tlm_ctx *c, const uint8_t *mat, const float *yhat, const float *g
     }
     HIPCHK(c, hipMemcpy(c->code, code.data(), code.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
     {
-        // blocks whose 64 codes are all 0 (the slab padding is coded PEC, so a
-        // block with padding never qualifies)
+        // bit 0: blocks whose 64 codes are all 0 (the slab padding is coded
+        // PEC, so a block with padding never qualifies); bit 2: blocks with a
+        // dielectric cell (they load their stubs with their pulses)
         const size_t nblk = (size_t)c->P / tlm::kWave;
         std::vector<uint8_t> ua((nblk + 3) / 4 * 4, 0); // padded for block_flag's dword loads
         for (size_t b = 0; b < nblk; ++b)
         {
             const uint16_t *q = &code[b * tlm::kWave];
             uint16_t any = 0;
+            bool diel = false;
             for (int l = 0; l < tlm::kWave; ++l)
+            {
                 any |= q[l];
-            ua[b] = any == 0;
+                diel = diel || (q[l] & 0xFFu) >= 2u;
+            }
+            ua[b] = (uint8_t)((any == 0 ? 1u : 0u) | (diel ? 4u : 0u));
         }
         int rc0 = upload(c, &c->uair, ua.data(), ua.size());
         c->fold_marked = false;
@@ -1353,7 +1387,8 @@ int tlm_set_tuning(tlm_ctx *c, const tlm_tuning *t)
         return fail(c, TLM_ERR_STATE, "context is a batch member");
     if (!c || !t || t->kernel != 0 || t->planes < 0 || t->block_waves < 0 ||
         t->block_waves > 4 || t->nontemporal < 0 || t->nontemporal > 2 || t->grid_blocks < 0 ||
-        t->fold < 0 || t->fold > 2)
+        t->fold < 0 || t->fold > 2 || t->occupancy < -1 || (t->occupancy > 0 && t->occupancy < 3) ||
+        t->occupancy > 8)
         return fail(c, TLM_ERR_ARG, "tlm_set_tuning: bad arguments");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->tune = *t;

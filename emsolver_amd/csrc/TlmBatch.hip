@@ -172,6 +172,9 @@ int rebuild(tlm_batch *b)
         a.tasks = tasks;
         a.groups = groups;
         a.rho = c->rho;
+        a.nmat = c->nmat;
+        a.mstrips = tlm::magic_div((unsigned)strips);
+        a.mnx = tlm::magic_div((unsigned)c->nx);
         d.buf[0] = M.buf[0];
         d.buf[1] = M.buf[1];
         d.tick0 = M.tick0;

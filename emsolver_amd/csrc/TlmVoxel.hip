@@ -660,7 +660,8 @@ __global__ void k_codes(const uint8_t *mat, int nx, int ny, int nz, long long nx
     }
 }
 
-// Per 64-cell block: 1 if all codes are 0 (air with no PEC neighbour).
+// Per 64-cell block: bit 0 if all codes are 0 (air with no PEC neighbour),
+// bit 2 if a cell is dielectric (material id >= 2 in the low byte of its code).
 __global__ void k_uair(const uint16_t *code, long long nblk, uint8_t *uair)
 {
     const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -668,12 +669,15 @@ __global__ void k_uair(const uint16_t *code, long long nblk, uint8_t *uair)
         return;
     const uint4 *q = reinterpret_cast<const uint4 *>(code + b * tlm::kWave);
     unsigned any = 0;
+    bool diel = false;
     for (int r = 0; r < tlm::kWave / 8; ++r)
     {
         const uint4 v = q[r];
         any |= v.x | v.y | v.z | v.w;
+        for (unsigned w : {v.x, v.y, v.z, v.w})
+            diel = diel || (w & 0xFFu) >= 2u || ((w >> 16) & 0xFFu) >= 2u;
     }
-    uair[b] = any == 0;
+    uair[b] = (uint8_t)((any == 0 ? 1u : 0u) | (diel ? 4u : 0u));
 }
 
 // Largest dielectric index (id - 2) of the map, -1 if none (16 cells per

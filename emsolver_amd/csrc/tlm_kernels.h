@@ -116,6 +116,43 @@ __device__ __forceinline__ void fold_sources(const Fold &f, long long c, float v
     }
 }
 
+// Division by a launch constant d (1 <= d < 2^31) of any x < 2^31 as a
+// multiply-high (Granlund and Montgomery): with l = ceil(log2 d) and
+// m = ceil(2^(31+l) / d) < 2^32, floor(x / d) = (x * m) >> (31 + l). The
+// wave's task and cell indices are split this way, with a few scalar or
+// vector instructions instead of a division sequence.
+struct Magic
+{
+    unsigned m;
+    int s;
+};
+__host__ inline Magic magic_div(unsigned d)
+{
+    int l = 0;
+    while ((1ull << l) < d)
+        ++l;
+    const unsigned long long num = 1ull << (31 + l);
+    return {(unsigned)((num + d - 1) / d), 31 + l};
+}
+__device__ __forceinline__ unsigned mdiv(unsigned x, Magic q)
+{
+    return (unsigned)(((unsigned long long)x * q.m) >> q.s);
+}
+
+// Workgroup -> task-group mapping of a launch of G step workgroups over
+// `groups` task groups, prepared on the host (see xcd_range below):
+// nr ranges (one per XCD, or G if fewer), per / extra groups per range,
+// gq / gr workgroups per range.
+struct XcdMap
+{
+    int nr, per, extra, gq, gr;
+};
+__host__ inline XcdMap xcd_map(int G, int groups)
+{
+    const int nr = G < 8 ? G : 8;
+    return {nr, groups / nr, groups % nr, G / nr, G % nr};
+}
+
 struct StepArgs
 {
     const char *A;        // incident pulses of step n: 12 planes of PB bytes
@@ -123,8 +160,8 @@ struct StepArgs
     float *Vs;            // stubs: 3 planes of P floats
     const uint16_t *code; // cell code (material + neighbour-PEC bits)
     const uint8_t *uair;  // per 64-cell block (slab-major): bit 0 all its codes are 0,
-                          // bit 1 it holds a folded source cell;
-                          // padded to a multiple of 4 bytes (block_flag)
+                          // bit 1 it holds a folded source cell, bit 2 a dielectric
+                          // cell; padded to a multiple of 4 bytes (block_flag)
     const float4 *props;  // per dielectric id: (yh, gh, den, 0)
     double *epart;        // energy partial per wave task
     // 16-bit formats only
@@ -143,6 +180,9 @@ struct StepArgs
     int tasks;           // strips * nzc
     int groups;          // task groups (one per workgroup's worth of waves)
     float rho;
+    int nmat;           // dielectric materials (props entries)
+    Magic mstrips, mnx; // division by strips and by nx
+    XcdMap xm;          // k_step's workgroup -> task-group mapping
     Fold fold; // FP32: soft sources added where the pulses are loaded
 };
 
@@ -209,6 +249,25 @@ struct XcdRange
 {
     int first, end, step;
 };
+// The same from the host-prepared map: no division when there are 8 ranges.
+__device__ __forceinline__ XcdRange xcd_range_map(int b, const XcdMap &m)
+{
+    int xcd, local;
+    if (m.nr == kNumXcd)
+    {
+        xcd = b & (kNumXcd - 1);
+        local = b >> 3;
+    }
+    else
+    {
+        xcd = b % m.nr;
+        local = b / m.nr;
+    }
+    const int gper = m.gq + (xcd < m.gr ? 1 : 0);
+    const int start = xcd * m.per + min(xcd, m.extra);
+    const int count = m.per + (xcd < m.extra ? 1 : 0);
+    return {start + local, start + count, gper};
+}
 __device__ __forceinline__ XcdRange xcd_range(int b, int G, int ngroups)
 {
     // ranges = min(8, G) so that every range has at least one workgroup
@@ -251,10 +310,10 @@ template <bool STUBS, bool ENERGY, bool NT, int PF, bool FOLD = false>
 __device__ __forceinline__ void strip_task(const StepArgs &a, int task, int lane, double &esum)
 {
     static_assert(PF == PF_F32, "the 16-bit formats use strip_task16");
-    const int tz = task / a.strips, t = task - tz * a.strips;
+    const int tz = (int)mdiv((unsigned)task, a.mstrips), t = task - tz * a.strips;
     const int f = t * kWave + lane; // cell inside the slab
     const bool act = f < a.nxy;
-    const int j = f / a.nx, i = f - j * a.nx;
+    const int j = (int)mdiv((unsigned)f, a.mnx), i = f - j * a.nx;
     const int k0 = tz * a.zc, k1 = min(a.nz, k0 + a.zc);
     const float rho = a.rho;
     const size_t slabB = (size_t)a.Pp * 48u;
@@ -282,11 +341,21 @@ __device__ __forceinline__ void strip_task(const StepArgs &a, int task, int lane
         // bit 0: plain-air block (no code load); bit 1: block holds a folded source
         const unsigned fl = block_flag(a.uair, (size_t)k * a.strips + t);
         const bool uair = (fl & 1u) != 0u;
+        const bool dblk = STUBS && (fl & 4u) != 0u;
+        const bool vtab = dblk && a.nmat <= kWave;
+        float tyh = 0.f, tden = 4.f;
+        if (vtab && lane < a.nmat)
+        {
+            const float4 pr = a.props[lane];
+            tyh = pr.x;
+            tden = pr.z;
+        }
+        float v[12];
+        float st[3] = {0.f, 0.f, 0.f};
+        float *S0 = a.Vs, *S1 = a.Vs + a.P, *S2 = a.Vs + 2 * a.P;
+        const unsigned bs = mo * 4u;
         if (act)
         {
-            float v[12];
-            // load order measured per variant: code first without stubs,
-            // pulses first with them
             if (!STUBS && !uair)
                 code = ldc(a.code, mo);
 #pragma unroll
@@ -294,9 +363,25 @@ __device__ __forceinline__ void strip_task(const StepArgs &a, int task, int lane
                 v[p] = ldo(Ak + p * 256, lo);
             if (STUBS && !uair)
                 code = ldc(a.code, mo);
+            if (dblk)
+            {
+                st[0] = ldo((const char *)S0, bs);
+                st[1] = ldo((const char *)S1, bs);
+                st[2] = ldo((const char *)S2, bs);
+            }
+        }
+        const unsigned m = code & 0xFFu;
+        float yhv = 0.f, denv = 4.f;
+        if (vtab)
+        {
+            const int idx = m >= 2u ? (int)(m - 2u) : 0;
+            yhv = __shfl(tyh, idx, kWave);
+            denv = __shfl(tden, idx, kWave);
+        }
+        if (act)
+        {
             if (FOLD && (fl & 2u))
                 fold_sources(a.fold, (long long)mo, v);
-            const unsigned m = code & 0xFFu;
             pec = (m == 1u);
             if (ENERGY)
             {
@@ -306,12 +391,13 @@ __device__ __forceinline__ void strip_task(const StepArgs &a, int task, int lane
             }
             if (STUBS && m >= 2u)
             {
-                float *S0 = a.Vs, *S1 = a.Vs + a.P, *S2 = a.Vs + 2 * a.P;
-                const unsigned bs = mo * 4u;
-                float st[3] = {ldo((const char *)S0, bs), ldo((const char *)S1, bs),
-                               ldo((const char *)S2, bs)};
-                const float4 pr = a.props[m - 2u];
-                scn::scatter_diel(v, st, pr.x, pr.z, o);
+                if (!vtab)
+                {
+                    const float4 pr = a.props[m - 2u];
+                    yhv = pr.x;
+                    denv = pr.z;
+                }
+                scn::scatter_diel(v, st, yhv, denv, o);
                 sto<false>((char *)S0, bs, st[0]);
                 sto<false>((char *)S1, bs, st[1]);
                 sto<false>((char *)S2, bs, st[2]);
@@ -530,10 +616,10 @@ __device__ __forceinline__ void stpair_at(char *buf, const Slab16 &s, int q, flo
 template <bool STUBS, bool ENERGY, int PF>
 __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int lane, double &esum)
 {
-    const int tz = task / a.strips, t = task - tz * a.strips;
+    const int tz = (int)mdiv((unsigned)task, a.mstrips), t = task - tz * a.strips;
     const int f = t * kWave + lane;
     const bool act = f < a.nxy;
-    const int j = f / a.nx, i = f - j * a.nx;
+    const int j = (int)mdiv((unsigned)f, a.mnx), i = f - j * a.nx;
     const int k0 = tz * a.zc, k1 = min(a.nz, k0 + a.zc);
     const float rho = a.rho;
     const bool hasL = act && lane > 0 && i > 0;
@@ -1144,7 +1230,7 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a, MonitorArgs m, int mon
     const int lane = threadIdx.x & (kWave - 1);
     const int wpb = blockDim.x / kWave;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const XcdRange gr = xcd_range(blockIdx.x, mon_from, a.groups);
+    const XcdRange gr = xcd_range_map(blockIdx.x, a.xm);
     double esum = 0.0;
     for (int g = gr.first; g < gr.end; g += gr.step)
     {

@@ -378,16 +378,18 @@ class TlmDevice:
                   "tlm_time_steps")
         return tot.value, (ker.value / nk.value if k > 0 and nk.value > 0 else None)
 
-    def set_tuning(self, kernel=0, planes=1, block_waves=0, nontemporal=2, grid_blocks=0, fold=2):
+    def set_tuning(self, kernel=0, planes=1, block_waves=0, nontemporal=2, grid_blocks=0, fold=2,
+                   occupancy=-1):
         t = _native.Tuning(int(kernel), int(planes), int(block_waves), int(nontemporal),
-                           int(grid_blocks), int(fold))
+                           int(grid_blocks), int(fold), int(occupancy))
         self._chk(self.lib.tlm_set_tuning(self.h, C.byref(t)), "tlm_set_tuning")
 
     def get_tuning(self):
         t = _native.Tuning()
         self._chk(self.lib.tlm_get_tuning(self.h, C.byref(t)), "tlm_get_tuning")
         return {"kernel": t.kernel, "planes": t.planes, "block_waves": t.block_waves,
-                "nontemporal": t.nontemporal, "grid_blocks": t.grid_blocks, "fold": t.fold}
+                "nontemporal": t.nontemporal, "grid_blocks": t.grid_blocks, "fold": t.fold,
+                "occupancy": t.occupancy}
 
     def launch_shape(self):
         r, z, g, t = C.c_int(), C.c_int(), C.c_int(), C.c_int()

@@ -226,6 +226,9 @@ typedef struct tlm_tuning {
                             for grids without stub cells and for grids with them of at most
                             4 M cells, where the launch it saves weighs more than the stub
                             kernel's own cost of the folded code */
+    int32_t occupancy;   /* most waves per SIMD the step kernel may hold (3..8), enforced by a
+                            share of the CU's LDS per workgroup; 0 = what its registers allow.
+                            Default -1: 5 for FP32 grids of at least 32 M cells, else none */
 } tlm_tuning;
 
 int tlm_set_tuning(tlm_ctx *ctx, const tlm_tuning *t);

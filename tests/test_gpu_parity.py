@@ -160,12 +160,14 @@ def test_free_space_64_full_run_tolerances():
 
 
 TUNINGS = [
-    {"kernel": 0, "planes": 1, "block_waves": 4, "nontemporal": 0, "grid_blocks": 0, "fold": 0},
-    {"kernel": 0, "planes": 0, "block_waves": 0, "nontemporal": 1, "grid_blocks": 0, "fold": 1},
-    {"kernel": 0, "planes": 3, "block_waves": 1, "nontemporal": 0, "grid_blocks": 5},
+    {"kernel": 0, "planes": 1, "block_waves": 4, "nontemporal": 0, "grid_blocks": 0, "fold": 0,
+     "occupancy": 0},
+    {"kernel": 0, "planes": 0, "block_waves": 0, "nontemporal": 1, "grid_blocks": 0, "fold": 1,
+     "occupancy": 8},
+    {"kernel": 0, "planes": 3, "block_waves": 1, "nontemporal": 0, "grid_blocks": 5, "occupancy": 3},
     {"kernel": 0, "planes": 1, "block_waves": 2, "nontemporal": 1, "grid_blocks": 17},
     {"kernel": 0, "planes": 64, "block_waves": 4, "nontemporal": 0, "grid_blocks": 8},
-    {"kernel": 0, "planes": 2, "block_waves": 4, "nontemporal": 2, "grid_blocks": 0},
+    {"kernel": 0, "planes": 2, "block_waves": 4, "nontemporal": 2, "grid_blocks": 0, "occupancy": 5},
 ]
 
 
@@ -178,7 +180,7 @@ def _run_tuned(sc, tuning, steps=None):
         return tlm.collect(dev, sc, steps)
 
 
-@pytest.mark.parametrize("tuning", TUNINGS, ids=lambda t: "z{planes}_w{block_waves}_nt{nontemporal}_g{grid_blocks}_f{}".format(t.get("fold", 2), **t))
+@pytest.mark.parametrize("tuning", TUNINGS, ids=lambda t: "z{planes}_w{block_waves}_nt{nontemporal}_g{grid_blocks}_f{}_o{}".format(t.get("fold", 2), t.get("occupancy", -1), **t))
 @pytest.mark.parametrize("shape", [(12, 9, 11), (300, 5, 7), (63, 7, 5), (2000, 2, 3)])
 def test_tunings_bitexact(shape, tuning):
     """Every launch shape gives the oracle's bits, including rows that start
@@ -218,15 +220,21 @@ def test_folded_and_launched_injection_agree(fold):
 
 def test_fold_tuning_rules():
     """tuning.fold: 0..2 accepted and read back (default 2), anything else
-    refused."""
+    refused; tuning.occupancy: -1 (automatic, the default), 0 or 3..8."""
     sc = _random_scene(20, 9, 7, seed=5, diel=False, steps=4)
     with tlm.open_scene(sc) as dev:
-        assert dev.get_tuning()["fold"] == 2
+        assert dev.get_tuning()["fold"] == 2 and dev.get_tuning()["occupancy"] == -1
         for f in (0, 1, 2):
             dev.set_tuning(fold=f)
             assert dev.get_tuning()["fold"] == f
         with pytest.raises(tlm.TlmError, match="tlm_set_tuning"):
             dev.set_tuning(fold=3)
+        for o in (-1, 0, 3, 8):
+            dev.set_tuning(occupancy=o)
+            assert dev.get_tuning()["occupancy"] == o
+        for o in (-2, 1, 2, 9):
+            with pytest.raises(tlm.TlmError, match="tlm_set_tuning"):
+                dev.set_tuning(occupancy=o)
 
 
 @pytest.mark.parametrize("every", [1, 3])
