@@ -629,10 +629,23 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
     bool pecPrev = false;
     unsigned mo = (unsigned)(k0 * a.Pp + f);
     Slab16 sp = {0, 0u, 8u, false, 1.0f, 1.0f};
+    // material table in two VGPRs of every lane (<= 64 materials), loaded once
+    // for the task's slabs: a dielectric cell picks its (yh, den) with a lane
+    // shuffle instead of a dependent load
+    const bool vtab = STUBS && a.nmat <= kWave;
+    float tyh = 0.f, tden = 4.f;
+    if (vtab && lane < a.nmat)
+    {
+        const float4 pr = a.props[lane];
+        tyh = pr.x;
+        tden = pr.z;
+    }
 
     for (int k = k0; k < k1; ++k, mo += (unsigned)a.Pp)
     {
         const Slab16 s = slab16(a, k, f);
+        // a block holding a dielectric cell loads its stubs with its pulses
+        const bool dblk = STUBS && (block_flag(a.uair, (size_t)k * a.strips + t) & 4u) != 0u;
         auto body = [&](auto h16_tag) {
             constexpr bool H = decltype(h16_tag)::value;
             constexpr unsigned ES = H ? 4u : 8u;
@@ -644,14 +657,33 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
             float mc = 0.f;
             bool pec = true;
             unsigned code = 0;
+            float v[12];
+            float st[3] = {0.f, 0.f, 0.f};
+            float *S0 = a.Vs, *S1 = a.Vs + a.P, *S2 = a.Vs + 2 * a.P;
+            const unsigned bs = mo * 4u;
             if (act)
             {
-                float v[12];
 #pragma unroll
                 for (int q = 0; q < 6; ++q)
                     ldpair<PF, H>(Ak + q * PQ, bo, s, v[2 * q], v[2 * q + 1]);
                 code = ldc(a.code, mo);
-                const unsigned m = code & 0xFFu;
+                if (dblk)
+                {
+                    st[0] = ldo((const char *)S0, bs);
+                    st[1] = ldo((const char *)S1, bs);
+                    st[2] = ldo((const char *)S2, bs);
+                }
+            }
+            const unsigned m = code & 0xFFu;
+            float yhv = 0.f, denv = 4.f;
+            if (vtab && dblk)
+            {
+                const int idx = m >= 2u ? (int)(m - 2u) : 0;
+                yhv = __shfl(tyh, idx, kWave);
+                denv = __shfl(tden, idx, kWave);
+            }
+            if (act)
+            {
                 pec = (m == 1u);
                 if (ENERGY)
                 {
@@ -661,12 +693,13 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
                 }
                 if (STUBS && m >= 2u)
                 {
-                    float *S0 = a.Vs, *S1 = a.Vs + a.P, *S2 = a.Vs + 2 * a.P;
-                    const unsigned bs = mo * 4u;
-                    float st[3] = {ldo((const char *)S0, bs), ldo((const char *)S1, bs),
-                                   ldo((const char *)S2, bs)};
-                    const float4 pr = a.props[m - 2u];
-                    scn::scatter_diel(v, st, pr.x, pr.z, o);
+                    if (!vtab)
+                    {
+                        const float4 pr = a.props[m - 2u];
+                        yhv = pr.x;
+                        denv = pr.z;
+                    }
+                    scn::scatter_diel(v, st, yhv, denv, o);
                     sto<false>((char *)S0, bs, st[0]);
                     sto<false>((char *)S1, bs, st[1]);
                     sto<false>((char *)S2, bs, st[2]);
