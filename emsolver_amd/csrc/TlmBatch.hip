@@ -34,6 +34,7 @@ struct Member
     long long tick0 = 0;
     char *buf[2] = {nullptr, nullptr};
     long long cells = 0;
+    bool range_err = false; // 16-bit storage: a value outgrew its slab scale
 };
 
 } // namespace
@@ -346,13 +347,13 @@ int queue_ticks(tlm_batch *b, int nticks)
 }
 
 // Waits for the queued ticks and brings every member's host-side state up
-// to date (current pulse buffer, step counters, running probe DFT, 16-bit
-// range flag).
+// to date (current pulse buffer, step counters, running probe DFT). A 16-bit
+// member whose range flag is set is marked (its results are invalid); the
+// others go on, and tlm_batch_remove reports it for that member only.
 int sync_members(tlm_batch *b)
 {
     BCHK(b, hipSetDevice(b->dev));
     BCHK(b, hipStreamSynchronize(b->stream));
-    int range_err = 0;
     for (Member &m : b->mem)
     {
         tlm_ctx *c = m.c;
@@ -364,17 +365,14 @@ int sync_members(tlm_batch *b)
         c->steps_run_hi = std::max(c->steps_run_hi, now);
         if (c->npr > 0 && c->nfreq > 0 && c->dft_done < now)
             tlmi::queue_probe_dft(c, c->dft_done, now, b->stream);
-        if (c->flag)
+        if (c->flag && !m.range_err)
         {
             int f = 0;
             BCHK(b, hipMemcpy(&f, c->flag, sizeof(int), hipMemcpyDeviceToHost));
-            range_err |= f;
+            m.range_err = f != 0;
         }
     }
     BCHK(b, hipStreamSynchronize(b->stream));
-    if (range_err)
-        return bfail(b, TLM_ERR_RANGE,
-                     "16-bit pulse storage: a value exceeded the range its slab scale allows");
     return TLM_OK;
 }
 
@@ -465,6 +463,9 @@ int tlm_batch_add(tlm_batch *b, tlm_ctx *c, int nsteps)
     if (c->nf > 0 || c->nh > 0 || c->pa >= 0 || c->frame_every > 0)
         return bfail(b, TLM_ERR_UNSUPPORTED,
                      "batch members record probes and ports (no Js, Huygens, plane or frames)");
+    if (c->every > 0)
+        return bfail(b, TLM_ERR_UNSUPPORTED,
+                     "batch members record no energy: tlm_set_energy_every(ctx, 0) before joining");
     const int n0 = c->next_step;
     if (n0 + nsteps > c->nsteps)
         return bfail(b, TLM_ERR_ARG, "steps beyond the member's schedule");
@@ -500,9 +501,17 @@ int tlm_batch_remove(tlm_batch *b, tlm_ctx *c)
     if (it == b->mem.end())
         return bfail(b, TLM_ERR_ARG, "context is not a member of this batch");
     int rc = sync_members(b);
+    const bool range_err = it->range_err;
     b->mem.erase(it);
     c->batch = nullptr;
     b->dirty = true;
+    if (rc == TLM_OK && range_err)
+    {
+        tlmi::fail(c, TLM_ERR_RANGE, "16-bit pulse storage: a value exceeded the range its slab scale allows");
+        return bfail(b, TLM_ERR_RANGE,
+                     "this member's 16-bit pulse storage exceeded its slab scale: its results are "
+                     "invalid (the other members are not affected)");
+    }
     return rc;
 }
 

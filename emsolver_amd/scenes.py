@@ -320,8 +320,8 @@ def batch_dipoles(count=256, lo=32, hi=128, steps=(500, 2000), seed=3, nprobes=2
     [lo, hi] cells, a centre-fed z wire dipole of length U[length]*nz over a
     seeded random-dielectric background (the dielectric-box generator scaled
     to the grid), steps uniform in [steps], `nprobes` seeded E probes, port
-    V/I recorded. No surface-current monitor (batched members record probes
-    and ports)."""
+    V/I recorded. No surface-current or energy monitor (batched members
+    record probes and ports)."""
     rng = np.random.default_rng(seed)
     out = []
     for g in range(count):
@@ -332,6 +332,7 @@ def batch_dipoles(count=256, lo=32, hi=128, steps=(500, 2000), seed=3, nprobes=2
         sc = dipole_scene(nx, ny, nz, length_frac=frac, steps=st,
                           background_boxes=background_boxes, seed=int(rng.integers(1 << 31)))
         sc.faces = False
+        sc.energy_every = 0
         sc.pulse = pulse
         probes = []
         while len(probes) < nprobes:

@@ -164,6 +164,7 @@ class TlmServer:
         finished = []   # [(TlmDevice, request, scene, admitted)]: left the batch, to read
         done = []
         tuned = False
+        retried = []    # requests solved again in FP32 after a 16-bit range error
         ticks = 0
         member_ticks = 0
         last_report = time.perf_counter()
@@ -239,10 +240,25 @@ class TlmServer:
                 if left > 0:
                     continue
                 r, sc, adm = live.pop(dev)
-                batch.remove(dev)
+                try:
+                    batch.remove(dev)
+                except tlm.TlmError as e:
+                    if e.code != tlm.ERR_RANGE:
+                        raise
+                    # its 16-bit storage outgrew a slab scale: its results are
+                    # invalid (the other members are not affected); it is
+                    # solved again, in FP32, ahead of the queue
+                    dev.close()
+                    retried.append(r.rid)
+                    with cond:
+                        ready[r.rid] = dataclasses.replace(sc, pulse="f32")
+                        cond.notify_all()
+                    queue.appendleft(r)
+                    continue
                 finished.append((dev, r, sc, adm))
         self.stats = {"ticks": ticks, "mean_members": member_ticks / max(1, ticks),
-                      "wall_s": time.perf_counter() - t0, "tuning": batch.get_tuning()}
+                      "wall_s": time.perf_counter() - t0, "tuning": batch.get_tuning(),
+                      "retried_f32": retried}
         batch.close()
         return done
 
@@ -305,6 +321,7 @@ def request_scene(dims, length_frac, steps, tol, seed, nprobes=2, name="req",
     sc = scenes.dipole_scene(nx, ny, nz, length_frac=length_frac, steps=steps,
                              background_boxes=8, seed=seed, device_setup=device_setup)
     sc.faces = False
+    sc.energy_every = 0  # batch members record probes and ports
     rng = np.random.default_rng(seed + 1)
     probes = []
     while len(probes) < nprobes:

@@ -24,8 +24,17 @@ ETA0 = np.float32(376.730313)     # TlmSolver.cpp:16
 PULSE_TYPES = {"f32": 0, "f16": 1, "bf16": 2}
 
 
+# return codes of the C ABI that callers act on
+ERR_RANGE = -7   # TLM_ERR_RANGE: 16-bit storage outgrew a slab scale
+
+
 class TlmError(RuntimeError):
-    pass
+    """A failed C ABI call; `code` is its TLM_ERR_* return code (None where
+    the error was found on the Python side)."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 def device_count() -> int:
@@ -131,7 +140,7 @@ class TlmDevice:
 
     def _chk(self, rc, what):
         if rc != 0:
-            raise TlmError(f"{what} failed ({rc}): {self.lib.tlm_last_error(self.h).decode()}")
+            raise TlmError(f"{what} failed ({rc}): {self.lib.tlm_last_error(self.h).decode()}", rc)
 
     def close(self):
         if getattr(self, "h", None):
@@ -657,15 +666,20 @@ class TlmBatch:
 
     def _chk(self, rc, what):
         if rc != 0:
-            raise TlmError(f"{what} failed ({rc}): {self.lib.tlm_batch_last_error(self.h).decode()}")
+            raise TlmError(f"{what} failed ({rc}): {self.lib.tlm_batch_last_error(self.h).decode()}",
+                           rc)
 
     def add(self, dev: TlmDevice, nsteps: int):
         self._chk(self.lib.tlm_batch_add(self.h, dev.h, int(nsteps)), "tlm_batch_add")
         self.members[dev.h.value] = dev
 
     def remove(self, dev: TlmDevice):
-        self._chk(self.lib.tlm_batch_remove(self.h, dev.h), "tlm_batch_remove")
+        """Takes dev out of the batch (also when the call reports an error:
+        TlmError with code ERR_RANGE means dev's own 16-bit results are
+        invalid, the batch and its other members go on)."""
+        rc = self.lib.tlm_batch_remove(self.h, dev.h)
         self.members.pop(dev.h.value, None)
+        self._chk(rc, "tlm_batch_remove")
 
     def run(self, nticks: int):
         self._chk(self.lib.tlm_batch_run(self.h, int(nticks)), "tlm_batch_run")
@@ -726,7 +740,9 @@ class TlmBatch:
 
 def run_scenes_batched(scene_list, device=0, tuning=None, autotune=0, block=64):
     """Run several scenes from step 0 in one batch; returns their SceneResults
-    (probes, ports, final state) in order."""
+    (probes, ports, final state) in order. Batch members record no energy, so
+    the scenes run with energy_every = 0."""
+    scene_list = [dataclasses.replace(sc, energy_every=0) for sc in scene_list]
     devs = [open_scene(sc, device) for sc in scene_list]
     with TlmBatch(device) as b:
         if tuning:

@@ -296,14 +296,20 @@ void tlm_batch_destroy(tlm_batch *b);
 const char *tlm_batch_last_error(const tlm_batch *b);
 
 /* ctx joins at its next step and runs nsteps more steps (within its
- * schedule), starting with the next tick. */
+ * schedule), starting with the next tick. Refused (TLM_ERR_UNSUPPORTED) for
+ * a context with Js, Huygens, plane or frame monitors or energy records
+ * (tlm_set_energy_every(ctx, 0) first). */
 int tlm_batch_add(tlm_batch *b, tlm_ctx *ctx, int nsteps);
-/* Waits for queued ticks; ctx leaves the batch and is a plain context again. */
+/* Waits for queued ticks; ctx leaves the batch and is a plain context again.
+ * TLM_ERR_RANGE (ctx removed all the same) if ctx's 16-bit storage exceeded
+ * a slab scale while batched: its results are invalid, the other members'
+ * are not affected. */
 int tlm_batch_remove(tlm_batch *b, tlm_ctx *ctx);
 /* Queues nticks ticks (asynchronous); finished members idle. */
 int tlm_batch_run(tlm_batch *b, int nticks);
 /* Waits for the queued ticks and updates every member (step counters,
- * running probe DFT); reports TLM_ERR_RANGE for a 16-bit member. */
+ * running probe DFT, the per-member 16-bit range flag that tlm_batch_remove
+ * reports). */
 int tlm_batch_sync(tlm_batch *b);
 /* Members and their remaining steps (as of the ticks queued); returns the
  * member count. */

@@ -18,11 +18,13 @@ def _mixed_scenes():
     out = scenes.batch_dipoles(count=4, lo=12, hi=40, steps=(30, 90), seed=11, nprobes=2)
     r = _random_scene(21, 9, 13, seed=4, rho=0.37, steps=40)
     r.faces = False
+    r.energy_every = 0
     out.append(r)
     out.append(dataclasses.replace(out[0], pulse="f16", fp32_halo=2, name="f16_member"))
     out.append(dataclasses.replace(out[1], pulse="bf16", fp32_halo=-1, name="bf16_member"))
     nd = _random_scene(17, 11, 8, seed=6, diel=False, steps=25)
     nd.faces = False
+    nd.energy_every = 0
     out.append(nd)
     return out
 
@@ -69,6 +71,7 @@ def test_batch_thin_members():
         sc = _random_scene(*shape, seed=n + 13, rho=0.37, pec_frac=0.0 if n < 8 else 0.1,
                            steps=20 + 3 * i)
         sc.faces = False
+        sc.energy_every = 0
         scs.append(sc)
     scs.append(dataclasses.replace(scs[3], pulse="f16", fp32_halo=1, name="thin_f16"))
     scs.append(scenes.batch_dipoles(count=1, lo=12, hi=20, steps=(30, 31), seed=3, nprobes=2)[0])
@@ -143,6 +146,10 @@ def test_batch_rules():
         bad = dataclasses.replace(sc, faces=True)
         with tlm.open_scene(bad) as d:
             with pytest.raises(tlm.TlmError, match="probes and ports"):
+                b.add(d, 5)
+        # energy records would stop while batched: refused, not dropped
+        with tlm.open_scene(dataclasses.replace(sc, energy_every=16)) as d:
+            with pytest.raises(tlm.TlmError, match="energy"):
                 b.add(d, 5)
         with tlm.open_scene(sc) as d:
             b.add(d, 5)

@@ -110,3 +110,70 @@ def test_served_storage_same_for_host_and_device_setup():
             for k in [k0, k1] + [int(c) // slab for c in sa.probes]:
                 assert g0 - sa.fp32_halo + serve.F16_MARGIN <= k <= g0 + gap - 1 + sa.fp32_halo - serve.F16_MARGIN
     assert n16 >= 1
+
+
+class _FakeDev:
+    def __init__(self, sc):
+        self.sc, self.nprobe, self.nport, self.closed = sc, 0, 0, False
+
+    def close(self):
+        self.closed = True
+
+
+class _FakeBatch:
+    """Stands in for tlm.TlmBatch: members finish after their steps; the
+    first FP16 member to leave reports a 16-bit range error, as
+    tlm_batch_remove does for a member whose pulses outgrew a slab scale."""
+    failed = []
+
+    def __init__(self, device):
+        self.left = {}
+
+    def set_tuning(self, **kw):
+        pass
+
+    def get_tuning(self):
+        return {}
+
+    def add(self, dev, steps):
+        self.left[dev] = steps
+
+    def run(self, ticks):
+        for d in self.left:
+            self.left[d] = max(0, self.left[d] - ticks)
+
+    def sync(self):
+        pass
+
+    def remaining(self):
+        return dict(self.left)
+
+    def remove(self, dev):
+        self.left.pop(dev)
+        if dev.sc.pulse == "f16" and not _FakeBatch.failed:
+            _FakeBatch.failed.append(dev)
+            raise serve.tlm.TlmError("tlm_batch_remove failed (-7)", serve.tlm.ERR_RANGE)
+
+    def close(self):
+        pass
+
+
+def test_server_reruns_a_range_failed_member_in_fp32(monkeypatch):
+    """A member whose 16-bit storage fails leaves the batch alone; the server
+    solves that request again in FP32 and every request completes once."""
+    _FakeBatch.failed = []
+    monkeypatch.setattr(serve.tlm, "TlmBatch", _FakeBatch)
+    monkeypatch.setattr(serve.tlm, "open_scene", lambda sc, device: _FakeDev(sc))
+    monkeypatch.setattr(serve.tlm, "device_memory", lambda device: (1 << 40, 1 << 40))
+    reqs = []
+    for rid in range(6):
+        sc = scenes.Scene(8, 8, 8, np.zeros(512, np.uint8), name=f"r{rid}",
+                          total_steps=40 + 10 * rid, pulse="f16" if rid % 2 else "f32")
+        reqs.append(serve.Request(rid, sc, 0.0))
+    srv = serve.TlmServer(block_ticks=16)
+    done = srv.serve(reqs, realtime=False)
+    assert sorted(c.rid for c in done) == list(range(6))
+    bad = _FakeBatch.failed[0]
+    assert bad.closed and srv.stats["retried_f32"] == [1]
+    byrid = {c.rid: c for c in done}
+    assert byrid[1].pulse == "f32" and byrid[3].pulse == "f16" and byrid[0].pulse == "f32"
