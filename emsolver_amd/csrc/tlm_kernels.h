@@ -290,6 +290,28 @@ __device__ __forceinline__ unsigned blk_off(unsigned g)
     return (g >> 6) * 3072u + (g & 63u) * 4u;
 }
 
+// Lane constants of a strip task, kept in bits 16.. of the cell-code word
+// (the code itself takes bits 0..13).
+enum : unsigned
+{
+    LF_BX0 = 1u << 16, // i == 0: x- face on the outer boundary
+    LF_BX1 = 1u << 17, // i == nx-1
+    LF_BY0 = 1u << 18, // j == 0
+    LF_BY1 = 1u << 19, // j == ny-1
+    LF_L = 1u << 20,   // x- neighbour in this wave (its pulse comes by shuffle)
+    LF_R = 1u << 21    // x+ neighbour in this wave
+};
+// a | b where the compiler cannot see that b is loop-invariant: the lane
+// flags then enter every slab's tests through the loaded code word, and the
+// masks derived from them live for one slab instead of being held in
+// (spilled) scalar registers across the z-march.
+__device__ __forceinline__ unsigned or_opaque(unsigned a, unsigned b)
+{
+    unsigned r;
+    asm("v_or_b32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 // Fused scatter + connect. A wavefront owns a strip of 64 consecutive cells
 // of one z-slab (lane = cell) and marches over zc slabs.
 //
@@ -398,12 +420,17 @@ __device__ __forceinline__ void strip_task(const StepArgs &a, int task, int lane
                     denv = pr.z;
                 }
                 scn::scatter_diel(v, st, yhv, denv, o);
+            }
+            else
+                scn::scatter_air(v, o);
+            if (dblk)
+            {
+                // every lane of the block stores its stubs (unchanged outside
+                // dielectric cells), so the stub lines are written whole
                 sto<false>((char *)S0, bs, st[0]);
                 sto<false>((char *)S1, bs, st[1]);
                 sto<false>((char *)S2, bs, st[2]);
             }
-            else
-                scn::scatter_air(v, o);
             if (pec)
             {
                 // a PEC cell keeps its incident pulses: o carries them
@@ -612,7 +639,8 @@ __device__ __forceinline__ void stpair_at(char *buf, const Slab16 &s, int q, flo
 }
 
 // The fused step of strip_task with pair blocks and per-slab formats (same
-// delivery rules, same FP32 arithmetic, values rounded where stored).
+// delivery rules, same FP32 arithmetic, values rounded where stored). Every
+// per-lane choice of value or slot is a select, not a branch.
 template <bool STUBS, bool ENERGY, int PF>
 __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int lane, double &esum)
 {
@@ -622,9 +650,10 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
     const int j = (int)mdiv((unsigned)f, a.mnx), i = f - j * a.nx;
     const int k0 = tz * a.zc, k1 = min(a.nz, k0 + a.zc);
     const float rho = a.rho;
-    const bool hasL = act && lane > 0 && i > 0;
-    const bool hasR = act && lane < kWave - 1 && i < a.nx - 1;
-    const bool bx0 = i == 0, bx1 = i == a.nx - 1, by0 = j == 0, by1 = j == a.ny - 1;
+    const unsigned lf = (i == 0 ? LF_BX0 : 0u) | (i == a.nx - 1 ? LF_BX1 : 0u) |
+                        (j == 0 ? LF_BY0 : 0u) | (j == a.ny - 1 ? LF_BY1 : 0u) |
+                        (act && lane > 0 && i > 0 ? LF_L : 0u) |
+                        (act && lane < kWave - 1 && i < a.nx - 1 ? LF_R : 0u);
     float zpo0 = 0.f, zpo1 = 0.f;
     bool pecPrev = false;
     unsigned mo = (unsigned)(k0 * a.Pp + f);
@@ -655,7 +684,6 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
             const unsigned bo = s.lo;
             float o[12];
             float mc = 0.f;
-            bool pec = true;
             unsigned code = 0;
             float v[12];
             float st[3] = {0.f, 0.f, 0.f};
@@ -674,7 +702,9 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
                     st[2] = ldo((const char *)S2, bs);
                 }
             }
-            const unsigned m = code & 0xFFu;
+            const unsigned cf = or_opaque(code, lf);
+            const unsigned m = cf & 0xFFu;
+            const bool pec = !act || m == 1u;
             float yhv = 0.f, denv = 4.f;
             if (vtab && dblk)
             {
@@ -684,7 +714,6 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
             }
             if (act)
             {
-                pec = (m == 1u);
                 if (ENERGY)
                 {
 #pragma unroll
@@ -700,29 +729,33 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
                         denv = pr.z;
                     }
                     scn::scatter_diel(v, st, yhv, denv, o);
-                    sto<false>((char *)S0, bs, st[0]);
-                    sto<false>((char *)S1, bs, st[1]);
-                    sto<false>((char *)S2, bs, st[2]);
                     mc = fmaxf(fmaxf(fabsf(st[0]), fabsf(st[1])), fabsf(st[2]));
                 }
                 else
                     scn::scatter_air(v, o);
-#pragma unroll
-                for (int p = 0; p < 12; ++p)
-                    mc = fmaxf(mc, fabsf(pec ? v[p] : o[p]));
-                if (pec)
+                if (dblk)
                 {
-                    // a PEC cell keeps its incident pulses: o carries them
-#pragma unroll
-                    for (int p = 0; p < 12; ++p)
-                        o[p] = v[p];
+                    // every lane of the block stores its stubs (unchanged
+                    // outside dielectric cells): the stub lines are written whole
+                    sto<false>((char *)S0, bs, st[0]);
+                    sto<false>((char *)S1, bs, st[1]);
+                    sto<false>((char *)S2, bs, st[2]);
                 }
             }
             else
             {
 #pragma unroll
                 for (int p = 0; p < 12; ++p)
-                    o[p] = 0.f;
+                    v[p] = 0.f;
+            }
+            // a PEC cell keeps its incident pulses: o carries them (an idle
+            // lane counts as PEC and sends zeros)
+#pragma unroll
+            for (int p = 0; p < 12; ++p)
+            {
+                const float vp = v[p];
+                o[p] = pec ? vp : o[p];
+                mc = fmaxf(mc, fabsf(o[p]));
             }
             const float L0 = __shfl_up(o[2], 1, kWave), L1 = __shfl_up(o[3], 1, kWave);
             const float R0 = __shfl_down(o[0], 1, kWave), R1 = __shfl_down(o[1], 1, kWave);
@@ -731,62 +764,52 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
                 // x and y faces: the value and the slot (this cell's own face,
                 // or the neighbour's opposite face) are selected per lane, so
                 // every face is one encode and one store
-                auto face = [&](int p, bool bnd, bool pecN, bool mine, float n0, float n1,
-                                unsigned own_q, unsigned nb_q, unsigned nb_g) {
-                    float x = o[p], y = o[p + 1];
-                    if (bnd)
-                    {
-                        x = rho * o[p];
-                        y = rho * o[p + 1];
-                    }
-                    else if (!pec && pecN)
-                    {
-                        x = -o[p];
-                        y = -o[p + 1];
-                    }
-                    else if (!pec && mine)
-                    {
-                        x = n0;
-                        y = n1;
-                    }
+                auto face = [&](int p, unsigned bbit, unsigned pbit, unsigned nbit, float n0,
+                                float n1, unsigned own_q, unsigned nb_q, unsigned nb_g) {
+                    const bool bnd = (cf & bbit) != 0u, pecN = (cf & pbit) != 0u;
+                    const bool mine = nbit != 0u && (cf & nbit) != 0u;
+                    const float o0 = o[p], o1 = o[p + 1];
+                    const float r0 = rho * o0, r1 = rho * o1, m0 = -o0, m1 = -o1;
+                    const bool take = !pec && mine, neg = !pec && pecN;
+                    float x = take ? n0 : o0, y = take ? n1 : o1;
+                    x = neg ? m0 : x;
+                    y = neg ? m1 : y;
+                    x = bnd ? r0 : x;
+                    y = bnd ? r1 : y;
                     const bool own = pec || bnd || pecN || mine;
-                    const unsigned off = own ? own_q * PQ + bo : nb_q * PQ + blk16(nb_g, ES);
-                    stpair<PF, H>(Bk, off, s, x, y);
+                    const unsigned ownoff = own_q * PQ + bo, nboff = nb_q * PQ + blk16(nb_g, ES);
+                    stpair<PF, H>(Bk, own ? ownoff : nboff, s, x, y);
                 };
                 // pair 0 (x-: ports 0,1), pair 1 (x+: ports 2,3): the left /
                 // right neighbour in this wave takes the pulse from a shuffle
-                face(0, bx0, (code & NB_XM) != 0u, hasL, L0, L1, 0u, 1u, (unsigned)f - 1u);
-                face(2, bx1, (code & NB_XP) != 0u, hasR, R0, R1, 1u, 0u, (unsigned)f + 1u);
+                face(0, LF_BX0, NB_XM, LF_L, L0, L1, 0u, 1u, (unsigned)f - 1u);
+                face(2, LF_BX1, NB_XP, LF_R, R0, R1, 1u, 0u, (unsigned)f + 1u);
                 // pair 2 (y-: ports 4,5), pair 3 (y+: ports 6,7): always pushed
-                face(4, by0, (code & NB_YM) != 0u, false, 0.f, 0.f, 2u, 3u,
-                     (unsigned)(f - a.nx));
-                face(6, by1, (code & NB_YP) != 0u, false, 0.f, 0.f, 3u, 2u,
-                     (unsigned)(f + a.nx));
+                face(4, LF_BY0, NB_YM, 0u, 0.f, 0.f, 2u, 3u, (unsigned)(f - a.nx));
+                face(6, LF_BY1, NB_YP, 0u, 0.f, 0.f, 3u, 2u, (unsigned)(f + a.nx));
                 // ---- z faces: pair 4 (z-: ports 8,9), pair 5 (z+: ports 10,11) ----
                 if (k > k0)
                 {
                     // pull: slab k-1's reflected z+ pulses are in registers
-                    if (pec)
-                        stpair<PF, H>(Bk + 4 * PQ, bo, s, o[8], o[9]);
-                    else
-                        stpair<PF, H>(Bk + 4 * PQ, bo, s, pecPrev ? -o[8] : zpo0,
-                                      pecPrev ? -o[9] : zpo1);
+                    const float n8 = -o[8], n9 = -o[9], nz0 = -zpo0, nz1 = -zpo1;
+                    float x = pecPrev ? n8 : zpo0, y = pecPrev ? n9 : zpo1;
+                    x = pec ? o[8] : x;
+                    y = pec ? o[9] : y;
+                    stpair<PF, H>(Bk + 4 * PQ, bo, s, x, y);
                     // slab k-1's z+ slot: its kept pulses if it is PEC (then
                     // zpo holds them), else what this cell sends down
-                    if (pecPrev)
-                        stpair_at<PF>(a.B, sp, 5, zpo0, zpo1);
-                    else
-                        stpair_at<PF>(a.B, sp, 5, pec ? -zpo0 : o[8], pec ? -zpo1 : o[9]);
+                    float u = pec ? nz0 : o[8], w = pec ? nz1 : o[9];
+                    u = pecPrev ? zpo0 : u;
+                    w = pecPrev ? zpo1 : w;
+                    stpair_at<PF>(a.B, sp, 5, u, w);
                 }
                 else if (k == 0)
                     stpair<PF, H>(Bk + 4 * PQ, bo, s, rho * o[8], rho * o[9]);
                 else
                 {
-                    const bool pecZ = (code & NB_ZM) != 0u;
-                    if (pec)
-                        stpair<PF, H>(Bk + 4 * PQ, bo, s, o[8], o[9]);
-                    else if (pecZ)
-                        stpair<PF, H>(Bk + 4 * PQ, bo, s, -o[8], -o[9]);
+                    const bool pecZ = (cf & NB_ZM) != 0u;
+                    if (pec || pecZ)
+                        stpair<PF, H>(Bk + 4 * PQ, bo, s, pec ? o[8] : -o[8], pec ? o[9] : -o[9]);
                     else
                         stpair_at<PF>(a.B, slab16(a, k - 1, f), 5, o[8], o[9]);
                 }
@@ -796,11 +819,10 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
                         stpair<PF, H>(Bk + 5 * PQ, bo, s, rho * o[10], rho * o[11]);
                     else
                     {
-                        const bool pecZ = (code & NB_ZP) != 0u;
-                        if (pec)
-                            stpair<PF, H>(Bk + 5 * PQ, bo, s, o[10], o[11]);
-                        else if (pecZ)
-                            stpair<PF, H>(Bk + 5 * PQ, bo, s, -o[10], -o[11]);
+                        const bool pecZ = (cf & NB_ZP) != 0u;
+                        if (pec || pecZ)
+                            stpair<PF, H>(Bk + 5 * PQ, bo, s, pec ? o[10] : -o[10],
+                                          pec ? o[11] : -o[11]);
                         else
                             stpair_at<PF>(a.B, slab16(a, k + 1, f), 4, o[10], o[11]);
                     }
