@@ -139,6 +139,34 @@ def gpu_run(scene, warmup, steps, tag, plain=False, kernel_every=None):
 GRAPH_TICKS = 32  # ticks per captured HIP graph in the batched and served lines
 
 
+def profile_batched(count=256, ticks=48):
+    """For counter profilers: the batched line's 256 grids in one batch,
+    autotuned, then `ticks` plain (not graph-captured) ticks; a progress line
+    per tick block. Reports no rate."""
+    from emsolver_amd import scenes, tlm
+    scs = scenes.batch_dipoles(count=count, seed=3)
+    devs = [tlm.open_scene(sc) for sc in scs]
+    with tlm.TlmBatch() as b:
+        b.set_tuning(graph_ticks=0)
+        for d, sc in zip(devs, scs):
+            b.add(d, sc.total_steps)
+        tuning = b.autotune(4)
+        b.sync()
+        done = 0
+        while done < ticks:
+            m = min(8, ticks - done)
+            b.run(m)
+            b.sync()
+            done += m
+            log(f"[bench] profile batched_{count}: {done} of {ticks} ticks")
+        for d in devs:
+            b.remove(d)
+    for d in devs:
+        d.close()
+    log(f"[bench] profile batched_{count}: {ticks} ticks, tuning {tuning}, "
+        f"{sum(sc.ncell for sc in scs)} cells per tick")
+
+
 def batched_run(count=256, sample_every=4):
     """256 independent dipole grids (axes log-uniform in [32,128], steps
     U[500,2000]) in one batch: autotuned launch shape, graph-captured blocks
@@ -198,6 +226,19 @@ def batched_run(count=256, sample_every=4):
         "port_absmax": probe_absmax,
     }
     out["speedup_vs_one_at_a_time"] = round(out["gcell_s"] / out["one_at_a_time_gcell_s"], 3)
+    # the mix's algorithmic bytes per cell-update; the rate above covers the
+    # whole run, the tail where few members are left included
+    bpc = sum(sc.ncell * scene_alg_bytes(sc) for sc in scs) / sum(sc.ncell for sc in scs)
+    out["alg_bytes_per_cell"] = round(bpc, 3)
+    out["roofline_frac"] = round(out["gcell_s"] * bpc / HBM_PEAK_GBS, 4)
+    path = os.path.join(ROOT, "profiles", f"k_step_batch_{count}.md")
+    if os.path.exists(path):
+        with open(path) as f:
+            m = re.search(r"traffic_bytes_per_tick:\s*([0-9.eE+]+)", f.read())
+        if m:
+            # HBM bytes of one tick with every member active (rocprofv3 PMC)
+            out["traffic_bytes_per_tick"] = float(m.group(1))
+            out["alg_bytes_per_tick"] = round(bpc * sum(sc.ncell for sc in scs))
     log(f"[bench] batched_{count}: {out['gcell_s']:.2f} Gcell/s aggregate in {wall:.1f}s "
         f"(tuning {tuning}); one at a time {out['one_at_a_time_gcell_s']:.2f} Gcell/s")
     return out
@@ -585,6 +626,9 @@ def main():
     ap.add_argument("--serve-host-setup", action="store_true",
                     help="served requests as host material maps uploaded at admission (default: "
                          "set-up operations voxelised on the device)")
+    ap.add_argument("--profile-batched", type=int, default=0, metavar="TICKS",
+                    help="only run TICKS plain ticks of the batched line's 256 grids, for "
+                         "counter profilers (no JSON line)")
     ap.add_argument("--serve-count", type=int, default=2000,
                     help="requests of the served line (the workload's full stream: 2000)")
     ap.add_argument("--serve-rate", type=float, default=50.0,
@@ -606,6 +650,9 @@ def main():
         GRAPH_TICKS = 0
     if args.precision_report:
         print(json.dumps(precision_report(steps=args.precision_steps)), flush=True)
+        return
+    if args.profile_batched > 0:
+        profile_batched(ticks=args.profile_batched)
         return
     if args.serve_only:
         print(json.dumps(served_run(args.serve_count, rate=args.serve_rate,
