@@ -136,6 +136,30 @@ int main()
     FarFieldData fa, fb;
     const bool ffa = a->computeFarField(fa), ffb = b->computeFarField(fb);
     check(ffa && ffb && same(fa.U, fb.U), "far field from the Huygens DFT identical", ffa, ffb);
+
+    // run() again on the stopped solver: the stop request stands until the
+    // next setup() (TlmSolver.cpp:293), so the second run advances nothing
+    // and leaves every result as the stopped run left it (the drop-in enters
+    // with curStep > 0 and must neither restart nor add to the sums)
+    const std::vector<float> pa = a->pulses(), sta = a->stubPulses();
+    const std::vector<float> vra = a->ports()[0].vRec, eza = a->probe(0).ez;
+    const std::vector<double> dra = a->probe(0).dftRe;
+    a->run();
+    check(a->currentStep() == S && a->isFinished() && !a->isRunning(),
+          "second run() after the stop: same step, finished", a->currentStep(), S);
+    check(same(a->pulses(), pa) && same(a->stubPulses(), sta), "second run(): pulses and stubs unchanged",
+          (long long)a->pulses().size(), (long long)pa.size());
+    check(same(a->ports()[0].vRec, vra) && same(a->probe(0).ez, eza) && same(a->probe(0).dftRe, dra),
+          "second run(): records and probe DFT unchanged", (long long)a->ports()[0].vRec.size(),
+          (long long)vra.size());
+    std::vector<int> sa2;
+    std::vector<float> va2, ja2;
+    a->getEnergyHistory(sa2, va2);
+    a->getJsDft(ja2);
+    check(same(sa2, sa) && same(va2, va), "second run(): energy history unchanged", (long long)sa2.size(),
+          (long long)sa.size());
+    check(same(ja2, ja), "second run(): finalised |Js| DFT unchanged", (long long)ja2.size(),
+          (long long)ja.size());
     std::printf(g_fail ? "STOPTEST FAILED (%d)\n" : "STOPTEST PASSED\n", g_fail);
     return g_fail ? 1 : 0;
 }

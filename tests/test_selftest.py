@@ -40,7 +40,9 @@ def test_selftest_on_gpu():
 def test_stop_mid_run_on_gpu():
     """requestStop() from a second thread (tests/selftest/tlm_stop.cpp): the
     run ends between blocks, run() finalises the DFTs and marks the solver
-    finished, and every result equals a run configured to end at that step."""
+    finished, and every result equals a run configured to end at that step;
+    a second run() on the stopped solver (the drop-in entered at curStep > 0,
+    the stop request still standing) changes nothing."""
     p = _run(STOP)
     assert "FAIL" not in p.stdout, p.stdout
     assert p.returncode == 0 and "STOPTEST PASSED" in p.stdout
