@@ -679,7 +679,7 @@ int tlm_create(const tlm_grid_desc *d, tlm_ctx **out)
     c->rho = d->rho;
     c->pf = d->pulse_type;
     if (c->pf != TLM_PULSE_F32)
-        c->tune.planes = 8; // 16-bit pair planes: 8 slabs per wave task measured best
+        c->tune.planes = 4; // 16-bit pair planes: 4 slabs per wave task measured best
     c->halo = d->fp32_halo < 0 ? -1 : (d->fp32_halo == 0 ? 8 : d->fp32_halo);
     c->PB = c->P * 4;
     c->inv_dl = -1.0f / d->dl;

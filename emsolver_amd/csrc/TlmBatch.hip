@@ -48,7 +48,7 @@ struct tlm_batch
     std::vector<Member> mem;
     long long tick = 0; // ticks queued so far (the device counter after they ran)
     long long *d_tick = nullptr;
-    tlm_batch_tuning tune = {1, 4, 0, 32, 8};
+    tlm_batch_tuning tune = {1, 4, 0, 32, 4};
     bool dirty = true;
     // device tables, rebuilt when the membership or the tuning changes
     tlm::BatchMember *d_mem = nullptr;
@@ -614,9 +614,9 @@ int tlm_batch_autotune(tlm_batch *b, int ticks_per_candidate, tlm_batch_tuning *
     for (int p : {1, 2, 4, 8})
         for (int w : {2, 4})
             for (int g : {0, 2048})
-                for (int p16 : {4, 8, 16})
+                for (int p16 : {2, 4, 8})
                 {
-                    if ((!have32 && p != 1) || (!have16 && p16 != 8))
+                    if ((!have32 && p != 1) || (!have16 && p16 != 4))
                         continue;
                     cands.push_back({p, w, g, 0, p16});
                 }

@@ -695,7 +695,7 @@ class TlmBatch:
         self.lib.tlm_batch_members(self.h, hs, ptr(rem, C.c_int32), n)
         return {self.members[hs[i]]: int(rem[i]) for i in range(n)}
 
-    def set_tuning(self, planes=1, block_waves=4, grid_blocks=0, graph_ticks=32, planes16=8):
+    def set_tuning(self, planes=1, block_waves=4, grid_blocks=0, graph_ticks=32, planes16=4):
         t = _native.BatchTuning(int(planes), int(block_waves), int(grid_blocks), int(graph_ticks),
                                 int(planes16))
         self._chk(self.lib.tlm_batch_set_tuning(self.h, C.byref(t)), "tlm_batch_set_tuning")

@@ -288,7 +288,7 @@ typedef struct tlm_batch_tuning {
     int32_t block_waves; /* wavefronts (64-cell strips) per workgroup, 1..4; 0 = 4 */
     int32_t grid_blocks; /* > 0: persistent grid of that many workgroups; 0 = one per group */
     int32_t graph_ticks; /* ticks per captured HIP graph (default 32); 0 = plain launches */
-    int32_t planes16;    /* z slabs per wave task of the 16-bit members (default 8); 0 = all */
+    int32_t planes16;    /* z slabs per wave task of the 16-bit members (default 4); 0 = all */
 } tlm_batch_tuning;
 
 int tlm_batch_create(int device, tlm_batch **out);
