@@ -22,7 +22,7 @@ struct tlm_ctx
     long long nxy = 0, Pp = 0, P = 0; // slab pitch roundup(nx*ny, 64), plane stride Pp*nz
     float dl = 0.f, rho = 0.f;
     // launch shape of the fused kernel
-    tlm_tuning tune = {0, 1, 4, 2, 0, 2, -1}; // non-temporal stores, folding, occupancy per grid
+    tlm_tuning tune = {0, 1, 4, 0, 0, 2, -1}; // folding and occupancy chosen per grid
     int strips = 0, zc = 0, nzc = 0, tasks = 0, groups = 0, blocks = 0, wpb = 4;
     size_t epart_cap = 0;
     // state

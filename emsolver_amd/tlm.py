@@ -387,7 +387,7 @@ class TlmDevice:
                   "tlm_time_steps")
         return tot.value, (ker.value / nk.value if k > 0 and nk.value > 0 else None)
 
-    def set_tuning(self, kernel=0, planes=1, block_waves=0, nontemporal=2, grid_blocks=0, fold=2,
+    def set_tuning(self, kernel=0, planes=1, block_waves=0, nontemporal=0, grid_blocks=0, fold=2,
                    occupancy=-1):
         t = _native.Tuning(int(kernel), int(planes), int(block_waves), int(nontemporal),
                            int(grid_blocks), int(fold), int(occupancy))

@@ -218,8 +218,8 @@ typedef struct tlm_tuning {
     int32_t kernel;      /* 0: one cell per lane, 64-cell strips per wavefront (only variant) */
     int32_t planes;      /* z slabs marched per wave task (default 1); 0 = automatic */
     int32_t block_waves; /* wavefronts per workgroup, 1..4; 0 = 4 */
-    int32_t nontemporal; /* non-temporal stores of the next-step pulses: 0 no, 1 yes, 2 when the
-                            grid has dielectric cells (default) */
+    int32_t nontemporal; /* non-temporal stores of the next-step pulses: 0 no (default), 1 yes,
+                            2 when the grid has dielectric cells */
     int32_t grid_blocks; /* > 0: persistent grid of that many workgroups; 0 = one per task group */
     int32_t fold;        /* FP32 with at most 32 source cells: add the soft sources inside the
                             step instead of a separate injection launch. 0 no, 1 yes, 2 (default)
