@@ -448,8 +448,9 @@ int queue_steps(tlm_ctx *c, int step0, int nsteps, int kernel_every, float *ms_k
     return TLM_OK;
 }
 
-// Chunked AoS <-> SoA conversion through a bounded staging buffer: pulses
-// (ncomp 12, any storage format, through pv) or FP32 stubs (ncomp 3, soa).
+// Chunked conversion between host cell order and the device layout through
+// a bounded staging buffer: pulses (ncomp 12, any storage format, through
+// pv) or FP32 stubs (ncomp 3, 64-cell stub blocks at soa).
 int convert(tlm_ctx *c, const float *host_in, float *host_out, const tlm::PulseView *pv,
             float *soa, int ncomp)
 {
@@ -477,8 +478,8 @@ int convert(tlm_ctx *c, const float *host_in, float *host_out, const tlm::PulseV
                 hipLaunchKernelGGL(tlm::k_pulses_in, dim3((unsigned)g), dim3(b), 0, c->stream, stage,
                                    *pv, c->nxy, c0, cnt);
             else
-                hipLaunchKernelGGL(tlm::k_aos_to_soa, dim3((unsigned)g), dim3(b), 0, c->stream, stage,
-                                   soa, c->P, ncomp, c->nxy, c->Pp, c0, cnt);
+                hipLaunchKernelGGL(tlm::k_stubs_in, dim3((unsigned)g), dim3(b), 0, c->stream, stage,
+                                   soa, c->nxy, c->Pp, c0, cnt);
         }
         else
         {
@@ -486,8 +487,8 @@ int convert(tlm_ctx *c, const float *host_in, float *host_out, const tlm::PulseV
                 hipLaunchKernelGGL(tlm::k_pulses_out, dim3((unsigned)g), dim3(b), 0, c->stream, *pv,
                                    stage, c->nxy, c0, cnt);
             else
-                hipLaunchKernelGGL(tlm::k_soa_to_aos, dim3((unsigned)g), dim3(b), 0, c->stream, soa,
-                                   stage, c->P, ncomp, c->nxy, c->Pp, c0, cnt);
+                hipLaunchKernelGGL(tlm::k_stubs_out, dim3((unsigned)g), dim3(b), 0, c->stream, soa,
+                                   stage, c->nxy, c->Pp, c0, cnt);
             hipError_t e = hipMemcpyAsync(host_out + c0 * ncomp, stage, cnt * ncomp * sizeof(float),
                                           hipMemcpyDeviceToHost, c->stream);
             if (e != hipSuccess)

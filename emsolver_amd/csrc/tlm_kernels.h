@@ -290,6 +290,15 @@ __device__ __forceinline__ unsigned blk_off(unsigned g)
     return (g >> 6) * 3072u + (g & 63u) * 4u;
 }
 
+// Stubs: 64-cell blocks like the pulses, block b (slab-padded cell index
+// >> 6) holding its cells' stubs 0, 1, 2 one after another (stub j of block
+// lane l at float b*192 + j*64 + l), so a wave's stubs are one 768-byte run.
+constexpr int kStubBlock = 3 * kWave;
+__device__ __forceinline__ float *stub_block(float *Vs, long long block)
+{
+    return Vs + (size_t)block * kStubBlock;
+}
+
 // Lane constants of a strip task, kept in bits 16.. of the cell-code word
 // (the code itself takes bits 0..13).
 enum : unsigned
@@ -374,8 +383,9 @@ __device__ __forceinline__ void strip_task(const StepArgs &a, int task, int lane
         }
         float v[12];
         float st[3] = {0.f, 0.f, 0.f};
-        float *S0 = a.Vs, *S1 = a.Vs + a.P, *S2 = a.Vs + 2 * a.P;
-        const unsigned bs = mo * 4u;
+        float *S0 = stub_block(a.Vs, (long long)k * a.strips + t), *S1 = S0 + kWave,
+              *S2 = S0 + 2 * kWave;
+        const unsigned bs = (unsigned)lane * 4u;
         if (act)
         {
             if (!STUBS && !uair)
@@ -687,8 +697,9 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
             unsigned code = 0;
             float v[12];
             float st[3] = {0.f, 0.f, 0.f};
-            float *S0 = a.Vs, *S1 = a.Vs + a.P, *S2 = a.Vs + 2 * a.P;
-            const unsigned bs = mo * 4u;
+            float *S0 = stub_block(a.Vs, (long long)k * a.strips + t), *S1 = S0 + kWave,
+                  *S2 = S0 + 2 * kWave;
+            const unsigned bs = (unsigned)lane * 4u;
             if (act)
             {
 #pragma unroll
@@ -1368,8 +1379,10 @@ static __global__ void k_probe_dft(const float *rec, int npr, const double *ctab
 
 // AoS (12 or 3 floats per cell, unpadded) <-> SoA padded planes, for the
 // cells [c0, c0 + count) of the unpadded index space. Stubs are always FP32.
-static __global__ void k_aos_to_soa(const float *aos, float *soa, long long P, int ncomp, long long nxy,
-                             long long Pp, long long c0, long long count)
+// Stubs (3 per cell) between host cell order and the device's 64-cell
+// stub blocks.
+static __global__ void k_stubs_in(const float *aos, float *Vs, long long nxy, long long Pp,
+                                  long long c0, long long count)
 {
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= count)
@@ -1377,12 +1390,13 @@ static __global__ void k_aos_to_soa(const float *aos, float *soa, long long P, i
     const long long cell = c0 + id;
     const long long k = cell / nxy;
     const long long pc = k * Pp + (cell - k * nxy);
-    for (int p = 0; p < ncomp; ++p)
-        soa[p * P + pc] = aos[id * ncomp + p];
+    float *b = stub_block(Vs, pc >> 6) + (pc & (kWave - 1));
+    for (int j = 0; j < 3; ++j)
+        b[j * kWave] = aos[id * 3 + j];
 }
 
-static __global__ void k_soa_to_aos(const float *soa, float *aos, long long P, int ncomp, long long nxy,
-                             long long Pp, long long c0, long long count)
+static __global__ void k_stubs_out(const float *Vs, float *aos, long long nxy, long long Pp,
+                                   long long c0, long long count)
 {
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= count)
@@ -1390,8 +1404,9 @@ static __global__ void k_soa_to_aos(const float *soa, float *aos, long long P, i
     const long long cell = c0 + id;
     const long long k = cell / nxy;
     const long long pc = k * Pp + (cell - k * nxy);
-    for (int p = 0; p < ncomp; ++p)
-        aos[id * ncomp + p] = soa[p * P + pc];
+    const float *b = stub_block(const_cast<float *>(Vs), pc >> 6) + (pc & (kWave - 1));
+    for (int j = 0; j < 3; ++j)
+        aos[id * 3 + j] = b[j * kWave];
 }
 
 // Pulses (12 per cell) in any storage format.
