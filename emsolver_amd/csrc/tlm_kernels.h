@@ -157,11 +157,12 @@ struct StepArgs
 {
     const char *A;        // incident pulses of step n: 12 planes of PB bytes
     char *B;              // incident pulses of step n+1
-    float *Vs;            // stubs: 3 planes of P floats
+    float *Vs;            // stubs: 3P floats in 64-cell blocks (stub_block)
     const uint16_t *code; // cell code (material + neighbour-PEC bits)
     const uint8_t *uair;  // per 64-cell block (slab-major): bit 0 all its codes are 0,
                           // bit 1 it holds a folded source cell, bit 2 a dielectric
-                          // cell; padded to a multiple of 4 bytes (block_flag)
+                          // cell, bits 3 / 4 a dielectric cell among cells 0-31 /
+                          // 32-63; padded to a multiple of 4 bytes (block_flag)
     const float4 *props;  // per dielectric id: (yh, gh, den, 0)
     double *epart;        // energy partial per wave task
     // 16-bit formats only
@@ -373,6 +374,9 @@ __device__ __forceinline__ void strip_task(const StepArgs &a, int task, int lane
         const unsigned fl = block_flag(a.uair, (size_t)k * a.strips + t);
         const bool uair = (fl & 1u) != 0u;
         const bool dblk = STUBS && (fl & 4u) != 0u;
+        // the lane's half of the block holds a dielectric cell: its stub
+        // lines are loaded and stored whole; an air half's are not touched
+        const bool dhalf = STUBS && ((fl >> (3 + (lane >> 5))) & 1u) != 0u;
         const bool vtab = dblk && a.nmat <= kWave;
         float tyh = 0.f, tden = 4.f;
         if (vtab && lane < a.nmat)
@@ -395,7 +399,7 @@ __device__ __forceinline__ void strip_task(const StepArgs &a, int task, int lane
                 v[p] = ldo(Ak + p * 256, lo);
             if (STUBS && !uair)
                 code = ldc(a.code, mo);
-            if (dblk)
+            if (dhalf)
             {
                 st[0] = ldo((const char *)S0, bs);
                 st[1] = ldo((const char *)S1, bs);
@@ -433,10 +437,11 @@ __device__ __forceinline__ void strip_task(const StepArgs &a, int task, int lane
             }
             else
                 scn::scatter_air(v, o);
-            if (dblk)
+            if (dhalf)
             {
-                // every lane of the block stores its stubs (unchanged outside
-                // dielectric cells), so the stub lines are written whole
+                // every lane of a half holding a dielectric cell stores its
+                // stubs (unchanged outside dielectric cells): the stub lines
+                // are written whole
                 sto<false>((char *)S0, bs, st[0]);
                 sto<false>((char *)S1, bs, st[1]);
                 sto<false>((char *)S2, bs, st[2]);
@@ -684,7 +689,11 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
     {
         const Slab16 s = slab16(a, k, f);
         // a block holding a dielectric cell loads its stubs with its pulses
-        const bool dblk = STUBS && (block_flag(a.uair, (size_t)k * a.strips + t) & 4u) != 0u;
+        const unsigned fl = STUBS ? block_flag(a.uair, (size_t)k * a.strips + t) : 0u;
+        const bool dblk = (fl & 4u) != 0u;
+        // the lane's half of the block holds a dielectric cell: its stub
+        // lines are loaded and stored whole; an air half's are not touched
+        const bool dhalf = ((fl >> (3 + (lane >> 5))) & 1u) != 0u;
         auto body = [&](auto h16_tag) {
             constexpr bool H = decltype(h16_tag)::value;
             constexpr unsigned ES = H ? 4u : 8u;
@@ -706,7 +715,7 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
                 for (int q = 0; q < 6; ++q)
                     ldpair<PF, H>(Ak + q * PQ, bo, s, v[2 * q], v[2 * q + 1]);
                 code = ldc(a.code, mo);
-                if (dblk)
+                if (dhalf)
                 {
                     st[0] = ldo((const char *)S0, bs);
                     st[1] = ldo((const char *)S1, bs);
@@ -744,10 +753,11 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
                 }
                 else
                     scn::scatter_air(v, o);
-                if (dblk)
+                if (dhalf)
                 {
-                    // every lane of the block stores its stubs (unchanged
-                    // outside dielectric cells): the stub lines are written whole
+                    // every lane of a half holding a dielectric cell stores its
+                    // stubs (unchanged outside dielectric cells): the stub
+                    // lines are written whole
                     sto<false>((char *)S0, bs, st[0]);
                     sto<false>((char *)S1, bs, st[1]);
                     sto<false>((char *)S2, bs, st[2]);
