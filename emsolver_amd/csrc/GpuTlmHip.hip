@@ -65,6 +65,18 @@ void choose_shape(tlm_ctx *c)
     c->wpb = std::max(1, std::min(4, c->tune.block_waves > 0 ? c->tune.block_waves : 4));
     c->groups = (c->tasks + c->wpb - 1) / c->wpb;
     c->blocks = c->tune.grid_blocks > 0 ? std::min(c->groups, c->tune.grid_blocks) : c->groups;
+    // One workgroup per group: XCD x takes chunks x, x + 8, ... of one
+    // z-slab's (or z-chunk's) groups, so that each XCD's share holds an even
+    // sample of the grid's depth and no XCD is left with most of its
+    // dielectric blocks (the step ends with the last XCD); a chunk's groups
+    // stay on one XCD, so the slots that two waves share (strip seams, rows)
+    // meet in one L2. A persistent grid keeps contiguous ranges.
+    c->xchunk = 0;
+    if (c->tune.grid_blocks <= 0)
+    {
+        c->xchunk = std::max(1, std::min(c->strips / c->wpb, (c->groups + 7) / 8));
+        c->blocks = 8 * tlm::xcd_chunk_map(c->groups, c->xchunk).cpx;
+    }
 }
 
 size_t energy_parts(const tlm_ctx *c)
@@ -211,7 +223,7 @@ int launch_step(tlm_ctx *c, int n, const tlm::MonitorArgs &m, int nmon, hipEvent
     a.nmat = c->nmat;
     a.mstrips = tlm::magic_div((unsigned)c->strips);
     a.mnx = tlm::magic_div((unsigned)c->nx);
-    a.xm = tlm::xcd_map(c->blocks, c->groups);
+    a.xm = c->xchunk > 0 ? tlm::xcd_chunk_map(c->groups, c->xchunk) : tlm::xcd_map(c->blocks, c->groups);
     a.fold = fold_of(c, n);
     if (c->pf == TLM_PULSE_F16)
         launch_pf<tlm::PF_F16>(c, a, energy, m, nmon, e0, e1);

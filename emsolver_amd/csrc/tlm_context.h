@@ -24,6 +24,7 @@ struct tlm_ctx
     // launch shape of the fused kernel
     tlm_tuning tune = {0, 1, 4, 0, 0, 2, -1}; // folding and occupancy chosen per grid
     int strips = 0, zc = 0, nzc = 0, tasks = 0, groups = 0, blocks = 0, wpb = 4;
+    int xchunk = 0; // > 0: groups per chunk of the XCD-interleaved map (choose_shape)
     size_t epart_cap = 0;
     // state
     int pf = TLM_PULSE_F32; // pulse storage format

@@ -146,11 +146,30 @@ __device__ __forceinline__ unsigned mdiv(unsigned x, Magic q)
 struct XcdMap
 {
     int nr, per, extra, gq, gr;
+    // chunk > 0: XCD x takes chunks x, x + 8, x + 16, ... of `chunk` groups
+    // each (a z-slab's groups, or a z-chunk's for the 16-bit march), so
+    // every XCD gets an even sample of the grid's depth; the launch then
+    // has 8 * cpx workgroups, one group each (those past `groups` idle)
+    int chunk, cpx, groups;
+    Magic mchunk;
 };
 __host__ inline XcdMap xcd_map(int G, int groups)
 {
     const int nr = G < 8 ? G : 8;
-    return {nr, groups / nr, groups % nr, G / nr, G % nr};
+    return {nr, groups / nr, groups % nr, G / nr, G % nr, 0, 0, groups, {0u, 0}};
+}
+// The interleaved map for a launch of one workgroup per group: returns the
+// map; its launch size is 8 * cpx.
+__host__ inline XcdMap xcd_chunk_map(int groups, int chunk)
+{
+    chunk = chunk < 1 ? 1 : chunk;
+    const int nch = (groups + chunk - 1) / chunk;
+    const int cpx = (nch + 7) / 8 * chunk;
+    XcdMap m = xcd_map(8 * cpx, groups);
+    m.chunk = chunk;
+    m.cpx = cpx;
+    m.mchunk = magic_div((unsigned)chunk);
+    return m;
 }
 
 struct StepArgs
@@ -253,6 +272,15 @@ struct XcdRange
 // The same from the host-prepared map: no division when there are 8 ranges.
 __device__ __forceinline__ XcdRange xcd_range_map(int b, const XcdMap &m)
 {
+    if (m.chunk > 0)
+    {
+        // workgroup b on XCD b % 8 takes local group l = b / 8 of that XCD:
+        // chunk q * 8 + x, offset l - q * chunk
+        const int x = b & (kNumXcd - 1), l = b >> 3;
+        const int q = (int)mdiv((unsigned)l, m.mchunk);
+        const int g = (q * kNumXcd + x) * m.chunk + (l - q * m.chunk);
+        return {g, g < m.groups ? g + 1 : g, 1};
+    }
     int xcd, local;
     if (m.nr == kNumXcd)
     {
