@@ -65,7 +65,7 @@ void choose_shape(tlm_ctx *c)
     c->wpb = std::max(1, std::min(4, c->tune.block_waves > 0 ? c->tune.block_waves : 4));
     c->groups = (c->tasks + c->wpb - 1) / c->wpb;
     c->blocks = c->tune.grid_blocks > 0 ? std::min(c->groups, c->tune.grid_blocks) : c->groups;
-    // One workgroup per group: XCD x takes chunks x, x + 8, ... of one
+    // One workgroup per group: XCD x takes chunks x, x + 8, ... of half a
     // z-slab's (or z-chunk's) groups, so that each XCD's share holds an even
     // sample of the grid's depth and no XCD is left with most of its
     // dielectric blocks (the step ends with the last XCD); a chunk's groups
@@ -74,7 +74,7 @@ void choose_shape(tlm_ctx *c)
     c->xchunk = 0;
     if (c->tune.grid_blocks <= 0)
     {
-        c->xchunk = std::max(1, std::min(c->strips / c->wpb, (c->groups + 7) / 8));
+        c->xchunk = std::max(1, std::min(c->strips / c->wpb / 2, (c->groups + 7) / 8));
         c->blocks = 8 * tlm::xcd_chunk_map(c->groups, c->xchunk).cpx;
     }
 }
