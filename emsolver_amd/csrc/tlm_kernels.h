@@ -350,6 +350,21 @@ __device__ __forceinline__ unsigned or_opaque(unsigned a, unsigned b)
     return r;
 }
 
+// Sum of the squares of a cell's 12 pulses in FP64 (each square is exact in
+// double): four independent partial sums, so the additions do not form one
+// dependent chain of twelve.
+__device__ __forceinline__ double cell_energy(const float v[12])
+{
+    double e[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        e[q] = (double)v[q] * (double)v[q];
+#pragma unroll
+    for (int p = 4; p < 12; ++p)
+        e[p & 3] = fma((double)v[p], (double)v[p], e[p & 3]);
+    return (e[0] + e[1]) + (e[2] + e[3]);
+}
+
 // Fused scatter + connect. A wavefront owns a strip of 64 consecutive cells
 // of one z-slab (lane = cell) and marches over zc slabs.
 //
@@ -448,11 +463,7 @@ __device__ __forceinline__ void strip_task(const StepArgs &a, int task, int lane
                 fold_sources(a.fold, (long long)mo, v);
             pec = (m == 1u);
             if (ENERGY)
-            {
-#pragma unroll
-                for (int p = 0; p < 12; ++p)
-                    esum += (double)v[p] * (double)v[p];
-            }
+                esum += cell_energy(v);
             if (STUBS && m >= 2u)
             {
                 if (!vtab)
@@ -763,11 +774,7 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
             if (act)
             {
                 if (ENERGY)
-                {
-#pragma unroll
-                    for (int p = 0; p < 12; ++p)
-                        esum += (double)v[p] * (double)v[p];
-                }
+                    esum += cell_energy(v);
                 if (STUBS && m >= 2u)
                 {
                     if (!vtab)
