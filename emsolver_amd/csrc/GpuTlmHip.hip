@@ -153,11 +153,15 @@ void launch_pf(tlm_ctx *c, const tlm::StepArgs &a, bool energy, const tlm::Monit
 
 // FP32 contexts with a few source cells may add them inside the step
 // instead of k_inject. That saves the injection launch: 64^3 free space 7.2
-// against 10.6 us a step, 128^3 dielectric boxes 40.9 against 44.0 us. But
-// the stub kernel compiled with the folded code runs slower on large grids
-// (256^3 boxes: 304.0 against 301.5 us a launch), so by default grids with
-// stub cells fold up to kFoldAutoCells cells and grids without any size
-// (512^3 free space: equal). tuning.fold: 0 never, 1 always, 2 by that rule.
+// against 10.6 us a step, 128^3 dielectric boxes 40.9 against 44.0 us. The
+// stub kernel compiled with the folded code takes fewer registers, which
+// lets more of its waves in: uncapped that is slower on mid-size grids
+// (256^3 boxes: 0.2859 against 0.2826 ms a launch), under the occupancy cap
+// of large grids it is faster (384^3 boxes 1.019 against 1.026 ms, 512^3
+// 2.558 against 2.569 ms, the injection launch saved on top). So by default
+// grids with stub cells fold up to kFoldAutoCells cells or when the step
+// runs capped, grids without stub cells at any size (512^3 free space:
+// equal). tuning.fold: 0 never, 1 always, 2 by that rule.
 constexpr long long kFoldAutoCells = 4LL << 20;
 bool fold_active(const tlm_ctx *c)
 {
@@ -165,8 +169,8 @@ bool fold_active(const tlm_ctx *c)
         return false;
     if (c->tune.fold == 1)
         return true;
-    return c->tune.fold == 2 &&
-           (!c->stubs || (long long)c->nx * c->ny * c->nz <= kFoldAutoCells);
+    return c->tune.fold == 2 && (!c->stubs || (long long)c->nx * c->ny * c->nz <= kFoldAutoCells ||
+                                 step_occupancy(c) > 0);
 }
 
 tlm::Fold fold_of(const tlm_ctx *c, int n)

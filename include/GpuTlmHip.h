@@ -223,9 +223,8 @@ typedef struct tlm_tuning {
     int32_t grid_blocks; /* > 0: persistent grid of that many workgroups; 0 = one per task group */
     int32_t fold;        /* FP32 with at most 32 source cells: add the soft sources inside the
                             step instead of a separate injection launch. 0 no, 1 yes, 2 (default)
-                            for grids without stub cells and for grids with them of at most
-                            4 M cells, where the launch it saves weighs more than the stub
-                            kernel's own cost of the folded code */
+                            for grids without stub cells, and for grids with them of at most
+                            4 M cells or whose step runs under the occupancy cap (below) */
     int32_t occupancy;   /* most waves per SIMD the step kernel may hold (3..8), enforced by a
                             share of the CU's LDS per workgroup; 0 = what its registers allow.
                             Default -1: 5 for FP32 grids of at least 32 M cells, else none */
