@@ -807,7 +807,8 @@ int tlm_upload(tlm_ctx *c, const uint8_t *mat, const float *yhat, const float *g
         // bit 0: blocks whose 64 codes are all 0 (the slab padding is coded
         // PEC, so a block with padding never qualifies); bit 2: blocks with a
         // dielectric cell (they load their stubs with their pulses); bits 3
-        // and 4: cells 0-31 / 32-63 of the block hold a dielectric cell
+        // and 4: cells 0-31 / 32-63 of the block hold a dielectric cell;
+        // bit 5 (kBlockClean): no cell is PEC or padding or has a PEC neighbour
         const size_t nblk = (size_t)c->P / tlm::kWave;
         std::vector<uint8_t> ua((nblk + 3) / 4 * 4, 0); // padded for block_flag's dword loads
         for (size_t b = 0; b < nblk; ++b)
@@ -815,13 +816,16 @@ int tlm_upload(tlm_ctx *c, const uint8_t *mat, const float *yhat, const float *g
             const uint16_t *q = &code[b * tlm::kWave];
             uint16_t any = 0;
             unsigned half = 0;
+            bool clean = true;
             for (int l = 0; l < tlm::kWave; ++l)
             {
                 any |= q[l];
                 if ((q[l] & 0xFFu) >= 2u)
                     half |= 8u << (l >> 5);
+                clean = clean && tlm::code_is_clean(q[l]);
             }
-            ua[b] = (uint8_t)((any == 0 ? 1u : 0u) | (half ? 4u : 0u) | half);
+            ua[b] = (uint8_t)((any == 0 ? 1u : 0u) | (half ? 4u : 0u) | half |
+                              (clean ? tlm::kBlockClean : 0u));
         }
         int rc0 = upload(c, &c->uair, ua.data(), ua.size());
         c->fold_marked = false;

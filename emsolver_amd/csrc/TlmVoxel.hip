@@ -662,7 +662,8 @@ __global__ void k_codes(const uint8_t *mat, int nx, int ny, int nz, long long nx
 
 // Per 64-cell block: bit 0 if all codes are 0 (air with no PEC neighbour),
 // bit 2 if a cell is dielectric (material id >= 2 in the low byte of its
-// code), bits 3 and 4 if one of cells 0-31 / 32-63 is.
+// code), bits 3 and 4 if one of cells 0-31 / 32-63 is, bit 5 (kBlockClean)
+// if no code is PEC (padding included) or has a neighbour-PEC bit.
 __global__ void k_uair(const uint16_t *code, long long nblk, uint8_t *uair)
 {
     const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -670,17 +671,22 @@ __global__ void k_uair(const uint16_t *code, long long nblk, uint8_t *uair)
         return;
     const uint4 *q = reinterpret_cast<const uint4 *>(code + b * tlm::kWave);
     unsigned any = 0, half = 0;
+    bool clean = true;
     for (int r = 0; r < tlm::kWave / 8; ++r)
     {
         const uint4 v = q[r];
         any |= v.x | v.y | v.z | v.w;
         bool diel = false;
         for (unsigned w : {v.x, v.y, v.z, v.w})
+        {
             diel = diel || (w & 0xFFu) >= 2u || ((w >> 16) & 0xFFu) >= 2u;
+            clean = clean && tlm::code_is_clean(w & 0xFFFFu) && tlm::code_is_clean(w >> 16);
+        }
         if (diel)
             half |= 8u << (r >> 2); // uint4 r holds cells 8r .. 8r+7
     }
-    uair[b] = (uint8_t)((any == 0 ? 1u : 0u) | (half ? 4u : 0u) | half);
+    uair[b] = (uint8_t)((any == 0 ? 1u : 0u) | (half ? 4u : 0u) | half |
+                        (clean ? tlm::kBlockClean : 0u));
 }
 
 // Largest dielectric index (id - 2) of the map, -1 if none (16 cells per

@@ -181,7 +181,9 @@ struct StepArgs
     const uint8_t *uair;  // per 64-cell block (slab-major): bit 0 all its codes are 0,
                           // bit 1 it holds a folded source cell, bit 2 a dielectric
                           // cell, bits 3 / 4 a dielectric cell among cells 0-31 /
-                          // 32-63; padded to a multiple of 4 bytes (block_flag)
+                          // 32-63, bit 5 (kBlockClean) no PEC cell, no padding and no
+                          // PEC neighbour bit; padded to a multiple of 4 bytes
+                          // (block_flag)
     const float4 *props;  // per dielectric id: (yh, gh, den, 0)
     double *epart;        // energy partial per wave task
     // 16-bit formats only
@@ -233,6 +235,15 @@ enum : unsigned
     NB_ZM = 1u << 12,
     NB_ZP = 1u << 13
 };
+
+// Block flag bit 5: every cell code of the 64-cell block has a material id
+// other than PEC (1) and no neighbour-PEC bit. The slab padding is coded PEC,
+// so such a block has all 64 cells inside the grid.
+constexpr unsigned kBlockClean = 1u << 5;
+__host__ __device__ inline bool code_is_clean(unsigned c)
+{
+    return (c & 0xFFu) != 1u && (c & 0x3F00u) == 0u;
+}
 
 // Flag byte b of the per-block flag array, read as the aligned dword holding
 // it through the constant address space: the index is wave-uniform and the
@@ -727,14 +738,20 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
     for (int k = k0; k < k1; ++k, mo += (unsigned)a.Pp)
     {
         const Slab16 s = slab16(a, k, f);
+        const unsigned fl = block_flag(a.uair, (size_t)k * a.strips + t);
         // a block holding a dielectric cell loads its stubs with its pulses
-        const unsigned fl = STUBS ? block_flag(a.uair, (size_t)k * a.strips + t) : 0u;
-        const bool dblk = (fl & 4u) != 0u;
+        const bool dblk = STUBS && (fl & 4u) != 0u;
         // the lane's half of the block holds a dielectric cell: its stub
         // lines are loaded and stored whole; an air half's are not touched
-        const bool dhalf = ((fl >> (3 + (lane >> 5))) & 1u) != 0u;
-        auto body = [&](auto h16_tag) {
+        const bool dhalf = STUBS && ((fl >> (3 + (lane >> 5))) & 1u) != 0u;
+        auto body = [&](auto h16_tag, auto clean_tag) {
             constexpr bool H = decltype(h16_tag)::value;
+            // a clean block (kBlockClean): every lane holds a grid cell, none
+            // is PEC and none has a PEC neighbour, so the PEC rules of every
+            // face drop out (the same values and slots as the general rules
+            // give such a block)
+            constexpr bool CL = decltype(clean_tag)::value;
+            const bool live = CL || act;
             constexpr unsigned ES = H ? 4u : 8u;
             constexpr unsigned PQ = 64u * ES; // bytes between pairs of a block
             const char *Ak = a.A + s.base;
@@ -748,7 +765,7 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
             float *S0 = stub_block(a.Vs, (long long)k * a.strips + t), *S1 = S0 + kWave,
                   *S2 = S0 + 2 * kWave;
             const unsigned bs = (unsigned)lane * 4u;
-            if (act)
+            if (live)
             {
 #pragma unroll
                 for (int q = 0; q < 6; ++q)
@@ -763,7 +780,7 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
             }
             const unsigned cf = or_opaque(code, lf);
             const unsigned m = cf & 0xFFu;
-            const bool pec = !act || m == 1u;
+            const bool pec = !CL && (!act || m == 1u);
             float yhv = 0.f, denv = 4.f;
             if (vtab && dblk)
             {
@@ -771,7 +788,7 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
                 yhv = __shfl(tyh, idx, kWave);
                 denv = __shfl(tden, idx, kWave);
             }
-            if (act)
+            if (live)
             {
                 if (ENERGY)
                     esum += cell_energy(v);
@@ -815,14 +832,14 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
             }
             const float L0 = __shfl_up(o[2], 1, kWave), L1 = __shfl_up(o[3], 1, kWave);
             const float R0 = __shfl_down(o[0], 1, kWave), R1 = __shfl_down(o[1], 1, kWave);
-            if (act)
+            if (live)
             {
                 // x and y faces: the value and the slot (this cell's own face,
                 // or the neighbour's opposite face) are selected per lane, so
                 // every face is one encode and one store
                 auto face = [&](int p, unsigned bbit, unsigned pbit, unsigned nbit, float n0,
                                 float n1, unsigned own_q, unsigned nb_q, unsigned nb_g) {
-                    const bool bnd = (cf & bbit) != 0u, pecN = (cf & pbit) != 0u;
+                    const bool bnd = (cf & bbit) != 0u, pecN = !CL && (cf & pbit) != 0u;
                     const bool mine = nbit != 0u && (cf & nbit) != 0u;
                     const float o0 = o[p], o1 = o[p + 1];
                     const float r0 = rho * o0, r1 = rho * o1, m0 = -o0, m1 = -o1;
@@ -848,22 +865,25 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
                 {
                     // pull: slab k-1's reflected z+ pulses are in registers
                     const float n8 = -o[8], n9 = -o[9], nz0 = -zpo0, nz1 = -zpo1;
-                    float x = pecPrev ? n8 : zpo0, y = pecPrev ? n9 : zpo1;
+                    // (in a clean block the cell below is not PEC either: it
+                    // would have set this cell's NB_ZM bit)
+                    const bool pp = !CL && pecPrev;
+                    float x = pp ? n8 : zpo0, y = pp ? n9 : zpo1;
                     x = pec ? o[8] : x;
                     y = pec ? o[9] : y;
                     stpair<PF, H>(Bk + 4 * PQ, bo, s, x, y);
                     // slab k-1's z+ slot: its kept pulses if it is PEC (then
                     // zpo holds them), else what this cell sends down
                     float u = pec ? nz0 : o[8], w = pec ? nz1 : o[9];
-                    u = pecPrev ? zpo0 : u;
-                    w = pecPrev ? zpo1 : w;
+                    u = pp ? zpo0 : u;
+                    w = pp ? zpo1 : w;
                     stpair_at<PF>(a.B, sp, 5, u, w);
                 }
                 else if (k == 0)
                     stpair<PF, H>(Bk + 4 * PQ, bo, s, rho * o[8], rho * o[9]);
                 else
                 {
-                    const bool pecZ = (cf & NB_ZM) != 0u;
+                    const bool pecZ = !CL && (cf & NB_ZM) != 0u;
                     if (pec || pecZ)
                         stpair<PF, H>(Bk + 4 * PQ, bo, s, pec ? o[8] : -o[8], pec ? o[9] : -o[9]);
                     else
@@ -875,7 +895,7 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
                         stpair<PF, H>(Bk + 5 * PQ, bo, s, rho * o[10], rho * o[11]);
                     else
                     {
-                        const bool pecZ = (cf & NB_ZP) != 0u;
+                        const bool pecZ = !CL && (cf & NB_ZP) != 0u;
                         if (pec || pecZ)
                             stpair<PF, H>(Bk + 5 * PQ, bo, s, pec ? o[10] : -o[10],
                                           pec ? o[11] : -o[11]);
@@ -892,10 +912,13 @@ __device__ __forceinline__ void strip_task16(const StepArgs &a, int task, int la
             zpo1 = o[11];
             pecPrev = pec;
         };
-        if (s.h16)
-            body(std::true_type{});
+        // the FP32 band (a few slabs near the sources) takes the general body
+        if (s.h16 && (fl & kBlockClean) != 0u)
+            body(std::true_type{}, std::true_type{});
+        else if (s.h16)
+            body(std::true_type{}, std::false_type{});
         else
-            body(std::false_type{});
+            body(std::false_type{}, std::false_type{});
         sp = s;
     }
 }
