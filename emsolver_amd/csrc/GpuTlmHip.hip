@@ -695,8 +695,11 @@ int tlm_create(const tlm_grid_desc *d, tlm_ctx **out)
     c->dl = d->dl;
     c->rho = d->rho;
     c->pf = d->pulse_type;
+    // 16-bit pair planes: slabs per wave task, measured (ms a launch at 3 / 4
+    // slabs): 64-box scenes 512^3 1.591 / 1.606, 384^3 0.681 / 0.687, 320^3
+    // 0.350 / 0.342, 8-box 256^3 0.193 / 0.183
     if (c->pf != TLM_PULSE_F32)
-        c->tune.planes = 4; // 16-bit pair planes: 4 slabs per wave task measured best
+        c->tune.planes = (long long)c->nx * c->ny * c->nz >= (48LL << 20) ? 3 : 4;
     c->halo = d->fp32_halo < 0 ? -1 : (d->fp32_halo == 0 ? 8 : d->fp32_halo);
     c->PB = c->P * 4;
     c->inv_dl = -1.0f / d->dl;

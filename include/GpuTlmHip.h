@@ -216,7 +216,9 @@ int tlm_copy_bandwidth(int device, size_t bytes, int iters, float *gbps, float *
  * are bit-identical for every setting). No counterpart in the reference. */
 typedef struct tlm_tuning {
     int32_t kernel;      /* 0: one cell per lane, 64-cell strips per wavefront (only variant) */
-    int32_t planes;      /* z slabs marched per wave task (default 1); 0 = automatic */
+    int32_t planes;      /* z slabs marched per wave task; 0 = automatic. Default 1 with FP32
+                            pulses; with 16-bit pulses 3 on grids of at least 48 M cells,
+                            else 4 */
     int32_t block_waves; /* wavefronts per workgroup, 1..4; 0 = 4 */
     int32_t nontemporal; /* non-temporal stores of the next-step pulses: 0 no (default), 1 yes,
                             2 when the grid has dielectric cells */
