@@ -49,6 +49,8 @@ int fail(tlm_ctx *c, int code, const std::string &msg)
 // zc slabs (tuning.planes; 0 picks zc for about 64K tasks). Task groups of
 // block_waves tasks go to workgroups; tuning.grid_blocks > 0 caps the grid
 // (persistent workgroups walking their XCD's range of groups).
+int step_occupancy(const tlm_ctx *c);
+
 void choose_shape(tlm_ctx *c)
 {
     c->strips = (int)(c->Pp / tlm::kWave);
@@ -62,7 +64,12 @@ void choose_shape(tlm_ctx *c)
     }
     c->nzc = (c->nz + c->zc - 1) / c->zc;
     c->tasks = c->strips * c->nzc;
-    c->wpb = std::max(1, std::min(4, c->tune.block_waves > 0 ? c->tune.block_waves : 4));
+    // waves per workgroup: 4, or 2 when the step runs under the occupancy cap
+    // (ms a launch at 4 / 2 / 1: 512^3 64 boxes 2.535 / 2.527 / 2.522, 384^3
+    // 1.069 / 1.057 / 1.059, 512^3 free space 2.180 / 2.180 / 2.183; uncapped
+    // 256^3 8 boxes 0.282 / 0.282 / 0.285)
+    const int auto_waves = c->pf == TLM_PULSE_F32 && step_occupancy(c) > 0 ? 2 : 4;
+    c->wpb = std::max(1, std::min(4, c->tune.block_waves > 0 ? c->tune.block_waves : auto_waves));
     c->groups = (c->tasks + c->wpb - 1) / c->wpb;
     c->blocks = c->tune.grid_blocks > 0 ? std::min(c->groups, c->tune.grid_blocks) : c->groups;
     // One workgroup per group: XCD x takes chunks x, x + 8, ... of half a

@@ -219,7 +219,8 @@ typedef struct tlm_tuning {
     int32_t planes;      /* z slabs marched per wave task; 0 = automatic. Default 1 with FP32
                             pulses; with 16-bit pulses 3 on grids of at least 48 M cells,
                             else 4 */
-    int32_t block_waves; /* wavefronts per workgroup, 1..4; 0 = 4 */
+    int32_t block_waves; /* wavefronts per workgroup, 1..4; 0 = automatic: 2 for FP32 steps
+                            under the occupancy cap (below), else 4 */
     int32_t nontemporal; /* non-temporal stores of the next-step pulses: 0 no (default), 1 yes,
                             2 when the grid has dielectric cells */
     int32_t grid_blocks; /* > 0: persistent grid of that many workgroups; 0 = one per task group */
