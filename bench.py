@@ -669,7 +669,7 @@ def main():
                                     nfreq=32, seed=1)
     log(f"[bench] {workload}: {scene.ncell} cells, {nboxes} boxes, f_diel "
         f"{scene.diel_fraction():.4f}, warmup {args.warmup}, steps {args.steps}")
-    # at 512^3 a launch takes ~2.7 ms: events on every launch cost nothing there
+    # at 512^3 a launch takes ~2.5 ms: events on every launch cost nothing there
     main_run = gpu_run(scene, args.warmup, args.steps, workload, plain=True,
                        kernel_every=1 if scene.ncell >= 512 ** 3 else KERNEL_EVERY)
     log(f"[bench] {workload}: {main_run['gcells']:.2f} Gcell/s wall, kernel "
