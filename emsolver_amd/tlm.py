@@ -387,10 +387,16 @@ class TlmDevice:
                   "tlm_time_steps")
         return tot.value, (ker.value / nk.value if k > 0 and nk.value > 0 else None)
 
-    def set_tuning(self, kernel=0, planes=1, block_waves=0, nontemporal=0, grid_blocks=0, fold=2,
-                   occupancy=-1):
-        t = _native.Tuning(int(kernel), int(planes), int(block_waves), int(nontemporal),
-                           int(grid_blocks), int(fold), int(occupancy))
+    def set_tuning(self, **changes):
+        """Sets the tlm_tuning fields named (kernel, planes, block_waves,
+        nontemporal, grid_blocks, fold, occupancy); the others keep their
+        current values. Results are bit-identical for every setting."""
+        cur = self.get_tuning()
+        unknown = set(changes) - set(cur)
+        if unknown:
+            raise TypeError(f"set_tuning: unknown fields {sorted(unknown)}")
+        cur.update(changes)
+        t = _native.Tuning(*(int(cur[f]) for f, _ in _native.Tuning._fields_))
         self._chk(self.lib.tlm_set_tuning(self.h, C.byref(t)), "tlm_set_tuning")
 
     def get_tuning(self):
@@ -695,9 +701,16 @@ class TlmBatch:
         self.lib.tlm_batch_members(self.h, hs, ptr(rem, C.c_int32), n)
         return {self.members[hs[i]]: int(rem[i]) for i in range(n)}
 
-    def set_tuning(self, planes=1, block_waves=4, grid_blocks=0, graph_ticks=32, planes16=4):
-        t = _native.BatchTuning(int(planes), int(block_waves), int(grid_blocks), int(graph_ticks),
-                                int(planes16))
+    def set_tuning(self, **changes):
+        """Sets the tlm_batch_tuning fields named (planes, block_waves,
+        grid_blocks, graph_ticks, planes16); the others keep their current
+        values."""
+        cur = self.get_tuning()
+        unknown = set(changes) - set(cur)
+        if unknown:
+            raise TypeError(f"set_tuning: unknown fields {sorted(unknown)}")
+        cur.update(changes)
+        t = _native.BatchTuning(*(int(cur[f]) for f, _ in _native.BatchTuning._fields_))
         self._chk(self.lib.tlm_batch_set_tuning(self.h, C.byref(t)), "tlm_batch_set_tuning")
 
     def get_tuning(self) -> dict:
